@@ -1,0 +1,173 @@
+"""Headline benchmark: integrals/s on the 1M-atom zincblende grid.
+
+Workload (synthetic, generated here with the zincblende module): 64x64x32
+conventional cells = 1,048,576 atoms on a 256x256x128 lattice grid, 20
+orbitals per atom, 8 states (4 p-like holes, 4 s-like electrons, sigma = 0.15
+box), --integrals=eeee,hhhh,ehhe,eheh (1024 integrals), --dielectric=10.89.
+
+One step = the compute stage of one coulombo run with the states already in
+HBM: interaction table + DCT-I, pair densities, the 36 prepares, every
+overlap, the results back on the host.  value = integrals / second of the
+complex64-pass + FP64-accumulate path (median step).  The FP64 path runs the
+same inputs for the accuracy report (max relative error of every integral).
+
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--cells NX NY NZ]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "coulombo-lattice_amd"))
+
+PEAK_HBM_GBS = 8000.0
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(w, quads, budget_s=20.0):
+    """Oracle (FP64 C restatement, OpenMP) on a bounded sample: one prepare of
+    the full grid plus the overlaps of that left product, scaled to the run."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import numpy as np
+    import oracle as O
+    import clat
+
+    X, Y, Z = w["dims"]
+    n = len(w["states"])
+    steps = clat.plan(quads, n)
+    lefts = sorted(set(int(s[1]) for s in steps))
+    t0 = time.time()
+    Gh = O.ghat(O.gtable(X, Y, Z, w["step"], 0.0, 10.89))
+    t_init = time.time() - t0
+    t0 = time.time()
+    rho = O.products(np.stack(w["states"]))
+    t_rho = time.time() - t0
+    L = lefts[0]
+    t0 = time.time()
+    F = np.zeros(X * Y * Z, np.complex128)
+    F[w["cells"]] = rho[L]
+    V = O.prepare(F.reshape(Z, Y, X), Gh).reshape(-1)[w["cells"]]
+    t_prep = time.time() - t0
+    t0 = time.time()
+    n_right = 0
+    for s in steps:
+        if int(s[1]) == L:
+            _ = np.sum(V * rho[int(s[3])])
+            n_right += 1
+    t_ovl = (time.time() - t0) / max(1, n_right)
+    total = t_init + t_rho + len(lefts) * t_prep + len(steps) * t_ovl
+    return {"value": len(quads) / total, "unit": "integrals/s", "cores": O.num_threads(), "kind": "port",
+            "sample": "FP64 oracle: DCT-I table, pair densities, 1 of %d prepares (%.1f s) and %d of its "
+                      "overlaps, extrapolated to the %d-integral run" % (len(lefts), t_prep, n_right, len(quads))}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--cells", type=int, nargs=3, default=[64, 64, 32])
+    ap.add_argument("--states", type=int, nargs=2, default=[4, 4])
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--schedule", choices=["shared", "rounds"], default="shared")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this engine runs on one GPU; --gpus %d is not supported "
+              "(the integral path needs no collectives: see DESIGN.md)" % args.gpus, file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import clat
+    import zincblende as zb
+
+    nx, ny, nz = args.cells
+    nh, ne = args.states
+    t0 = time.time()
+    w = zb.workload(nx, ny, nz, nh, ne, config=1, sigma_frac=0.15)
+    X, Y, Z = w["dims"]
+    quads = clat.pattern_enumerate("eeee,hhhh,ehhe,eheh", nh + ne, nh)
+    log("workload: %d atoms, grid %dx%dx%d, %d states, %d integrals (%.1f s)"
+        % (len(w["cells"]), X, Y, Z, nh + ne, len(quads), time.time() - t0))
+    sched = clat.SCHED_SHARED if args.schedule == "shared" else clat.SCHED_ROUNDS
+
+    def make(prec):
+        eng = clat.Engine((X, Y, Z), prec, sched)
+        eng.set_atoms(w["cells"], 20)
+        for s in w["states"]:
+            eng.add_state(s)
+        return eng
+
+    inter = clat.InteractionSimple(w["step"], 0.0, 10.89)
+
+    def step(eng):
+        eng.initialize(inter)
+        return eng.compute_integrals(quads)
+
+    res = {}
+    for name, prec in (("fp32", clat.PREC_FP32), ("fp64", clat.PREC_FP64)):
+        if name == "fp64" and args.no_fp64:
+            continue
+        eng = make(prec)
+        for _ in range(args.warmup):
+            step(eng)
+        eng.set_kernel_timing(name == "fp32")
+        eng.reset_kernel_stats()
+        times = []
+        vals = None
+        t_all = time.perf_counter()
+        for _ in range(args.steps):
+            t = time.perf_counter()
+            vals = step(eng)
+            times.append(time.perf_counter() - t)
+        t_all = time.perf_counter() - t_all
+        res[name] = dict(times=times, vals=vals, stats=eng.kernel_stats(), stage=eng.stage_ms().tolist(),
+                         total=t_all)
+        log("%s: median step %.2f ms  stages(ms) table %.2f rho %.2f prepares %.2f overlaps %.2f"
+            % (name, 1e3 * float(np.median(times)), *res[name]["stage"][:4]))
+        eng.close()
+
+    fp32 = res["fp32"]
+    ms = 1e3 * fp32["total"] / args.steps
+    value = len(quads) / (fp32["total"] / args.steps)
+    # roofline of the dominant FFT pass kernel, from HIP events around each launch
+    stats = fp32["stats"]
+    dom = max(stats, key=lambda k: stats[k][1])
+    n_l, t_ms, byts = stats[dom]
+    avg_ms = t_ms / max(1, n_l)
+    achieved = (byts / max(1, n_l)) / (avg_ms * 1e-3) / 1e9
+    out = {
+        "metric": "integrals/s, 1M-atom grid, FP32+FP64-acc vs FP64 at 1e-5 rel; FFT HBM GB/s",
+        "value": value, "unit": "integrals/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "c64 passes + f64 accumulate", "data": "synthetic (seeded zincblende, see zincblende.py)",
+        "config": {"workload": "zincblende %dx%dx%d cells = %d atoms, grid %dx%dx%d, %d states, "
+                               "eeee,hhhh,ehhe,eheh" % (nx, ny, nz, len(w["cells"]), X, Y, Z, nh + ne),
+                   "integrals": len(quads), "schedule": args.schedule},
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                     "frac": achieved / PEAK_HBM_GBS, "traffic": None, "avg_launch_ms": avg_ms,
+                     "algorithmic_bytes_per_launch": byts / max(1, n_l)},
+        "kernel_ms_per_step": {k: v[1] / args.steps for k, v in stats.items() if v[0]},
+    }
+    if "fp64" in res:
+        v64 = res["fp64"]["vals"]
+        v32 = fp32["vals"]
+        rel = np.abs(v32 - v64) / np.abs(v64)
+        out["fp64_ms_per_step"] = 1e3 * res["fp64"]["total"] / args.steps
+        out["speedup_vs_fp64"] = out["fp64_ms_per_step"] / ms
+        out["max_rel_err_fp32_vs_fp64"] = float(np.max(rel))
+    if not args.no_cpu_baseline:
+        try:
+            out["cpu_baseline"] = cpu_baseline(w, quads)
+        except Exception as ex:  # reported, never fatal to the GPU number
+            out["cpu_baseline"] = {"value": None, "error": str(ex)}
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

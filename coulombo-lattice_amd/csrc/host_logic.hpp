@@ -1,0 +1,303 @@
+// Host-side logic of the integral path: grid rounding, lattice detection,
+// atom -> cell indexing, integral patterns, the vertex-cover planner and the
+// export layout.  Header-only C++17, shared by the C-ABI library and the
+// command-line drivers.  Errors are reported as clat::host::Error.
+#pragma once
+
+#include <algorithm>
+#include <cctype>
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <limits>
+#include <set>
+#include <stdexcept>
+#include <string>
+#include <utility>
+#include <vector>
+
+namespace clat {
+namespace host {
+
+struct Error : std::runtime_error {
+  int code;
+  Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+// ---------------------------------------------------------------------------
+// Round::up<p1, p2, ...>(n, c)  (src/Round.hpp:20-27, 38-49): smallest
+// integer >= n of the form c * p1^a1 * p2^a2 * ...
+// ---------------------------------------------------------------------------
+inline int64_t round_up_rec(int64_t n, int64_t c, const int* p, int np) {
+  if (np == 1) {
+    while (c < n) c *= p[0];
+    return c;
+  }
+  int64_t best = round_up_rec(n, c, p + 1, np - 1);
+  while (c < n) {
+    c *= p[0];
+    best = std::min(best, round_up_rec(n, c, p + 1, np - 1));
+  }
+  return best;
+}
+
+inline int round_up(int n, int c, const std::vector<int>& primes) {
+  if (primes.empty() || c < 1) throw Error(1, "invalid rounding request");
+  for (int p : primes)
+    if (p <= 1) throw Error(1, "rounding factors must be > 1");
+  const int64_t r = round_up_rec(n, c, primes.data(), (int)primes.size());
+  if (r > std::numeric_limits<int>::max()) throw Error(1, "rounded size overflows int");
+  return (int)r;
+}
+
+inline int round235(int n) { return round_up(n, 1, {2, 3, 5}); }
+inline int round2357(int n) { return round_up(n, 1, {2, 3, 5, 7}); }
+
+// ---------------------------------------------------------------------------
+// Lattice detection (src/Broadcaster.cpp:8-43).  Floating-point gcd of the
+// coordinate differences with the reference's 1e-10 / 1e-2 thresholds.
+// ---------------------------------------------------------------------------
+inline double float_gcd(double a, double b) {
+  if (a < b) std::swap(a, b);
+  while (true) {
+    const double r = std::fmod(a, b);
+    if (r < 1.0e-10) return b;
+    if (r < 1.0e-2) return 0.0;
+    a = b;
+    b = r;
+  }
+}
+
+struct AxisGrid {
+  int size;
+  double origin, step;
+};
+
+// values: one coordinate of every atom (copied, sorted here)
+inline AxisGrid compute_axis(std::vector<double> v) {
+  if (v.empty()) throw Error(1, "no atoms");
+  std::sort(v.begin(), v.end());
+  double step = NAN;
+  for (size_t a = 1; a < v.size(); ++a) {
+    const double next = v[a] - v[a - 1];
+    if (next > 1.0e-10) {
+      step = std::isnan(step) ? next : float_gcd(step, next);
+      if (step == 0) throw Error(1, "could not find common grid step");
+    }
+  }
+  AxisGrid g;
+  g.origin = v[0];
+  g.step = step;
+  if (std::isnan(step)) {
+    // every atom shares this coordinate: one grid plane.  The reference
+    // evaluates lrint(0/NaN) here, which has no defined value.
+    g.size = 1;
+    g.step = 1.0;
+  } else {
+    const long s = std::lrint((v.back() - v.front()) / step) + 1;
+    if (s <= 0 || s > std::numeric_limits<int>::max()) throw Error(1, "dimensions are too large");
+    g.size = (int)s;
+  }
+  return g;
+}
+
+struct Lattice {
+  int raw[3];
+  int pad[3];
+  double origin[3];
+  double step[3];
+};
+
+inline Lattice detect_lattice(const double* xyz, int64_t n) {
+  Lattice L;
+  for (int d = 0; d < 3; ++d) {
+    std::vector<double> v((size_t)n);
+    for (int64_t a = 0; a < n; ++a) v[(size_t)a] = xyz[3 * a + d];
+    AxisGrid g = compute_axis(std::move(v));
+    L.raw[d] = g.size;
+    L.origin[d] = g.origin;
+    L.step[d] = g.step;
+    L.pad[d] = round235(g.size);  // Broadcaster::round4FFT (Broadcaster.cpp:136-143)
+  }
+  const double cells = (double)L.pad[0] * L.pad[1] * L.pad[2];
+  if (cells >= 2147483647.0) throw Error(1, "dimensions are too large");
+  return L;
+}
+
+// Broadcaster.cpp:76-88 at one rank.
+inline void cell_indices(const double* xyz, int64_t n, const int dim[3], const double origin[3],
+                         const double step[3], int32_t* cell) {
+  for (int64_t a = 0; a < n; ++a) {
+    const long ix = std::lrint((xyz[3 * a + 0] - origin[0]) / step[0]);
+    const long iy = std::lrint((xyz[3 * a + 1] - origin[1]) / step[1]);
+    const long iz = std::lrint((xyz[3 * a + 2] - origin[2]) / step[2]);
+    if (ix < 0 || iy < 0 || iz < 0 || ix >= dim[0] || iy >= dim[1] || iz >= dim[2])
+      throw Error(1, "atom outside the grid");
+    cell[a] = (int32_t)((iz * (long)dim[1] + iy) * dim[0] + ix);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Integral pattern list (src/Pattern.hpp).  Characters: '*', digits 1-9,
+// 'h', 'e', any other letter a free variable bound on first use.  As in the
+// reference only the first four characters are read; a shorter pattern reads
+// the string terminator as a fourth free variable.
+// ---------------------------------------------------------------------------
+class Pattern {
+ public:
+  Pattern(const std::string& description, int hole_count) : holes_(hole_count) {
+    std::string::size_type start = 0, comma;
+    while ((comma = description.find(',', start)) != std::string::npos) {
+      add(description.substr(start, comma - start));
+      start = comma + 1;
+    }
+    add(description.substr(start));
+  }
+
+  bool match(int i0, int i1, int i2, int i3) const {
+    for (const std::string& p : patterns_)
+      if (match_one(p, i0, i1, i2, i3)) return true;
+    return false;
+  }
+
+ private:
+  int holes_;
+  std::vector<std::string> patterns_;
+
+  static void check_char(char c) {
+    const bool ok = (c == '*') || (std::isalnum((unsigned char)c) && c != '0');
+    if (!ok) {
+      char msg[64];
+      std::snprintf(msg, sizeof msg, "invalid character '%c' in pattern", c);
+      throw Error(1, msg);
+    }
+  }
+
+  void add(const std::string& p) {
+    if (p.empty()) throw Error(1, "invalid empty pattern");
+    for (char c : p) check_char(c);
+    patterns_.push_back(p);
+  }
+
+  bool letter(char c, int i, int* assign) const {
+    if (c == '*') return true;
+    if (c >= '1' && c <= '9') return i == (c - '0');
+    if (c == 'h') return i <= holes_;
+    if (c == 'e') return i > holes_;
+    const int k = (unsigned char)c & 127;
+    if (assign[k]) return i == assign[k];
+    assign[k] = i;
+    return true;
+  }
+
+  bool match_one(const std::string& p, int i0, int i1, int i2, int i3) const {
+    int assign[128] = {0};
+    auto ch = [&](size_t k) { return k < p.size() ? p[k] : '\0'; };
+    return letter(ch(0), i0, assign) && letter(ch(1), i1, assign) && letter(ch(2), i2, assign) &&
+           letter(ch(3), i3, assign);
+  }
+};
+
+// coulombo.cpp:135-148
+inline std::vector<int32_t> enumerate_integrals(const Pattern& pat, int n) {
+  std::vector<int32_t> q;
+  for (int i1 = 1; i1 <= n; ++i1)
+    for (int i2 = 1; i2 <= n; ++i2)
+      for (int i3 = 1; i3 <= n; ++i3)
+        for (int i4 = 1; i4 <= n; ++i4)
+          if (pat.match(i1, i2, i3, i4)) {
+            q.push_back(i1);
+            q.push_back(i2);
+            q.push_back(i3);
+            q.push_back(i4);
+          }
+  return q;
+}
+
+// ---------------------------------------------------------------------------
+// Greedy vertex cover (src/Graph.hpp:49-91).
+// ---------------------------------------------------------------------------
+inline std::vector<int> vertex_cover(int nv, const std::set<std::pair<int, int>>& edges) {
+  std::set<int> cover;
+  for (const auto& e : edges)
+    if (e.first == e.second) cover.insert(e.first);
+  std::vector<std::set<int>> nb((size_t)nv);
+  for (const auto& e : edges)
+    if (e.first != e.second && !cover.count(e.first) && !cover.count(e.second)) {
+      nb[e.first].insert(e.second);
+      nb[e.second].insert(e.first);
+    }
+  auto cmp = [&nb](int v, int w) {
+    const size_t vs = nb[v].size(), ws = nb[w].size();
+    return (vs < ws) || (vs == ws && v < w);
+  };
+  std::set<int, decltype(cmp)> queue(cmp);
+  for (int v = 0; v < nv; ++v)
+    if (!nb[v].empty()) queue.insert(v);
+  while (!queue.empty()) {
+    int v = *queue.begin();
+    if (nb[v].size() == 1) v = *nb[v].begin();
+    else v = *queue.rbegin();
+    queue.erase(v);
+    cover.insert(v);
+    for (int w : nb[v]) {
+      if (queue.erase(w)) {
+        nb[w].erase(v);
+        if (!nb[w].empty()) queue.insert(w);
+      }
+    }
+  }
+  return std::vector<int>(cover.begin(), cover.end());
+}
+
+// ---------------------------------------------------------------------------
+// MasterPlanner (src/Planner.hpp:65-145).
+// ---------------------------------------------------------------------------
+struct PlanProduct {
+  int product;
+  int conj;
+};
+struct PlanStep {
+  int id;
+  PlanProduct left, right;
+};
+
+// Planner.hpp:134-144 (1-based states; iL >= iR after the swap)
+inline PlanProduct plan_product(int iL, int iR) {
+  int conj = 0;
+  if (iL < iR) {
+    conj = 1;
+    std::swap(iL, iR);
+  }
+  return PlanProduct{iL * (iL - 1) / 2 + iR - 1, conj};
+}
+
+inline std::vector<PlanStep> make_plan(const int32_t* quads, int64_t n, int n_states) {
+  if (n > std::numeric_limits<int>::max()) throw Error(1, "too many integrals to compute");
+  std::vector<PlanStep> steps((size_t)n);
+  const int P = n_states * (n_states + 1) / 2;
+  std::set<std::pair<int, int>> edges;
+  for (int64_t k = 0; k < n; ++k) {
+    const int32_t* q = quads + 4 * k;
+    for (int j = 0; j < 4; ++j)
+      if (q[j] < 1 || q[j] > n_states) throw Error(1, "integral state index out of range");
+    steps[(size_t)k] = PlanStep{(int)k, plan_product(q[0], q[3]), plan_product(q[1], q[2])};
+    edges.insert({steps[(size_t)k].left.product, steps[(size_t)k].right.product});
+  }
+  const std::vector<int> cv = vertex_cover(P, edges);
+  std::vector<char> in_cover((size_t)P, 0);
+  for (int v : cv) in_cover[(size_t)v] = 1;
+  for (PlanStep& s : steps)
+    if (!in_cover[(size_t)s.left.product]) std::swap(s.left, s.right);
+  std::sort(steps.begin(), steps.end(), [](const PlanStep& a, const PlanStep& b) {
+    if (a.left.product < b.left.product) return true;
+    if (a.left.product > b.left.product) return false;
+    if (a.right.product < b.right.product) return true;
+    if (a.right.product > b.right.product) return false;
+    return (a.left.conj == a.right.conj) && (b.left.conj != b.right.conj);
+  });
+  return steps;
+}
+
+}  // namespace host
+}  // namespace clat

@@ -1,0 +1,597 @@
+// HIP kernels of the engine.  Every kernel here is launched from engine.hip.
+//
+// Pass kernels of one prepare() (CoulombCalculator.cpp:105-163).  With the
+// shared schedule the eight half-shift rounds are the eight parity classes
+// of the 2X x 2Y x 2Z frequency grid; each pass turns a line of length L into
+// two parity sub-lines (forward) or folds them back (inverse):
+//   forward, parity p: FFT_L(x(n) e^{-i pi p n / L})  -> frequency 2k+p
+//   inverse:           y(n) = IFFT_L(Y_0)(n) + e^{+i pi n / L} IFFT_L(Y_1)(n)
+// With the rounds schedule each pass carries one sub-line and the phase of the
+// round is applied in the first x pass and removed in the last one, as the
+// reference's copy/phase sweeps do (CoulombCalculator.cpp:113-126,148-161).
+#pragma once
+
+#include "fft_device.hpp"
+
+namespace clat {
+
+// ---------------------------------------------------------------------------
+// x passes: contiguous lines, line-major LDS tile of C lines.
+// ---------------------------------------------------------------------------
+
+// Forward x pass.  F: nlines lines of L; out: nlines lines of nsub*L.
+// nsub == 2: out[l][2k+p] = FFT(x e^{-i pi p n/L})[k].
+// nsub == 1: out[l][k] = FFT(x * ph)[k], ph = (kx ? hx[n] : 1) * hy[y]^ky * hz[z]^kz
+//            with l = y + Y*z (the conj phase of round (kx,ky,kz)).
+template <typename T, int EMAX>
+__global__ __launch_bounds__(NT) void k_xfwd(const C2<T>* __restrict__ F, C2<T>* __restrict__ out,
+                                             AxisPlan p, const C2<T>* __restrict__ tw,
+                                             const C2<T>* __restrict__ hx, int nlines, int C, int lgC,
+                                             int nsub, int kx, int ky, int kz,
+                                             const C2<T>* __restrict__ hy, const C2<T>* __restrict__ hz,
+                                             int Y) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  C2<T>* buf = reinterpret_cast<C2<T>*>(smem_raw);
+  const int L = p.L;
+  const int l0 = blockIdx.x * C;
+  const int nC = min(C, nlines - l0);
+  const int tot = C * L;
+  for (int e = threadIdx.x; e < tot; e += NT) {
+    const int c = (int)fdiv((uint32_t)e, p.l_magic);
+    const int i = e - c * L;
+    C2<T> x = mk<T>((T)0, (T)0);
+    if (c < nC) x = F[(size_t)(l0 + c) * L + i];
+    if (nsub == 2) {
+      buf[(2 * c) * L + i] = x;
+      buf[(2 * c + 1) * L + i] = cmul<T>(x, hx[i]);
+    } else {
+      C2<T> ph = kx ? hx[i] : mk<T>((T)1, (T)0);
+      if (ky | kz) {
+        const int l = l0 + c;
+        const int z = (c < nC) ? l / Y : 0;
+        const int y = (c < nC) ? l - z * Y : 0;
+        if (ky) ph = cmul<T>(ph, hy[y]);
+        if (kz) ph = cmul<T>(ph, hz[z]);
+      }
+      buf[c * L + i] = cmul<T>(x, ph);
+    }
+  }
+  __syncthreads();
+  const int nl = nsub * C;
+  lds_fft<false, EMAX, true, T>(buf, tw, p, nl, lgC + nsub - 1);
+  if (nsub == 2) {
+    const int tot2 = 2 * tot;
+    for (int e = threadIdx.x; e < tot2; e += NT) {
+      const int c = (int)fdiv((uint32_t)e, p.l2_magic);
+      const int f = e - c * 2 * L;
+      if (c < nC) out[(size_t)(l0 + c) * 2 * L + f] = buf[(2 * c + (f & 1)) * L + (f >> 1)];
+    }
+  } else {
+    for (int e = threadIdx.x; e < tot; e += NT) {
+      const int c = (int)fdiv((uint32_t)e, p.l_magic);
+      if (c < nC) out[(size_t)l0 * L + e] = buf[e];
+    }
+  }
+}
+
+// Inverse x pass.  in: nlines lines of nsub*L; V: nlines lines of L.
+// nsub == 2: V[l][n] = IFFT(Y_even)(n) + e^{i pi n/L} IFFT(Y_odd)(n).
+// nsub == 1: V[l][n] (+)= IFFT(Y)(n) * conj(ph) (accumulate when acc != 0).
+template <typename T, int EMAX>
+__global__ __launch_bounds__(NT) void k_xinv(const C2<T>* __restrict__ in, C2<T>* __restrict__ V,
+                                             AxisPlan p, const C2<T>* __restrict__ tw,
+                                             const C2<T>* __restrict__ hx, int nlines, int C, int lgC,
+                                             int nsub, int kx, int ky, int kz,
+                                             const C2<T>* __restrict__ hy, const C2<T>* __restrict__ hz,
+                                             int Y, int acc) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  C2<T>* buf = reinterpret_cast<C2<T>*>(smem_raw);
+  const int L = p.L;
+  const int l0 = blockIdx.x * C;
+  const int nC = min(C, nlines - l0);
+  const int tot = C * L;
+  if (nsub == 2) {
+    const int tot2 = 2 * tot;
+    for (int e = threadIdx.x; e < tot2; e += NT) {
+      const int c = (int)fdiv((uint32_t)e, p.l2_magic);
+      const int f = e - c * 2 * L;
+      C2<T> x = mk<T>((T)0, (T)0);
+      if (c < nC) x = in[(size_t)(l0 + c) * 2 * L + f];
+      buf[(2 * c + (f & 1)) * L + (f >> 1)] = x;
+    }
+  } else {
+    for (int e = threadIdx.x; e < tot; e += NT) {
+      const int c = (int)fdiv((uint32_t)e, p.l_magic);
+      buf[e] = (c < nC) ? in[(size_t)l0 * L + e] : mk<T>((T)0, (T)0);
+    }
+  }
+  __syncthreads();
+  lds_fft<true, EMAX, true, T>(buf, tw, p, nsub * C, lgC + nsub - 1);
+  for (int e = threadIdx.x; e < tot; e += NT) {
+    const int c = (int)fdiv((uint32_t)e, p.l_magic);
+    const int n = e - c * L;
+    if (c >= nC) continue;
+    C2<T> v;
+    if (nsub == 2) {
+      v = cadd<T>(buf[(2 * c) * L + n], cmulc<T>(buf[(2 * c + 1) * L + n], hx[n]));
+    } else {
+      C2<T> ph = kx ? hx[n] : mk<T>((T)1, (T)0);
+      if (ky | kz) {
+        const int l = l0 + c;
+        const int z = l / Y;
+        const int y = l - z * Y;
+        if (ky) ph = cmul<T>(ph, hy[y]);
+        if (kz) ph = cmul<T>(ph, hz[z]);
+      }
+      v = cmulc<T>(buf[c * L + n], ph);
+    }
+    const size_t g = (size_t)l0 * L + e;
+    if (acc) v = cadd<T>(V[g], v);
+    V[g] = v;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Column passes (y and z axes): a block holds C consecutive columns (the
+// contiguous fx index) of one outer index o; element (c, i) of the line is at
+// base + o*os + i*is + col0 + c.  Index-major LDS tile.
+// ---------------------------------------------------------------------------
+
+enum ColMode { COL_FWD = 0, COL_CONV = 1, COL_INV = 2 };
+
+struct ColArgs {
+  int W;          // number of columns
+  int C, lgC;     // columns per block
+  long long in_os, in_is, out_os, out_is;
+  int nsub;       // 2: shared schedule (parity sub-lines), 1: one round
+  // COL_CONV only: kernel table lookup
+  int zpar;       // z parity when nsub == 1
+  int gxm, gxa;   // fx = (col)*gxm + gxa
+  int gym, gya;   // fy = o*gym + gya
+  int X2, Y2, Z2; // 2*Xi, 2*Yi, 2*Zi (reflection r(f) = min(f, 2N - f))
+  int GX1, GY1;   // table pitch: (Xi+1), (Yi+1)
+};
+
+template <typename T, int EMAX, int MODE>
+__global__ __launch_bounds__(NT) void k_col(const C2<T>* in, C2<T>* out, AxisPlan p,
+                                            const C2<T>* __restrict__ tw,
+                                            const C2<T>* __restrict__ h, ColArgs a,
+                                            const T* __restrict__ G) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  C2<T>* buf = reinterpret_cast<C2<T>*>(smem_raw);
+  const int L = p.L;
+  const int C = a.C, lgC = a.lgC;
+  const int col0 = blockIdx.x * C;
+  const int o = blockIdx.y;
+  const int nC = min(C, a.W - col0);
+  const int nsub = a.nsub;
+  const int nl = nsub * C;
+  const int lgnl = lgC + nsub - 1;
+  const C2<T>* src = in + (size_t)o * a.in_os + col0;
+  C2<T>* dst = out + (size_t)o * a.out_os + col0;
+
+  if (MODE == COL_INV) {
+    const int tot = nsub * L * C;
+    for (int e = threadIdx.x; e < tot; e += NT) {
+      const int c = e & (C - 1);
+      const int r = e >> lgC;
+      const int s = r & (nsub - 1);
+      const int k = r >> (nsub - 1);
+      buf[k * nl + s * C + c] = (c < nC) ? src[(size_t)r * a.in_is + c] : mk<T>((T)0, (T)0);
+    }
+  } else {
+    const int tot = L * C;
+    for (int e = threadIdx.x; e < tot; e += NT) {
+      const int c = e & (C - 1);
+      const int i = e >> lgC;
+      C2<T> x = (c < nC) ? src[(size_t)i * a.in_is + c] : mk<T>((T)0, (T)0);
+      if (nsub == 2) {
+        buf[i * nl + c] = x;
+        buf[i * nl + C + c] = cmul<T>(x, h[i]);
+      } else {
+        buf[i * nl + c] = x;
+      }
+    }
+  }
+  __syncthreads();
+
+  if (MODE != COL_INV) lds_fft<false, EMAX, false, T>(buf, tw, p, nl, lgnl);
+
+  if (MODE == COL_FWD) {
+    const int tot = nsub * L * C;
+    for (int e = threadIdx.x; e < tot; e += NT) {
+      const int c = e & (C - 1);
+      const int r = e >> lgC;
+      const int s = r & (nsub - 1);
+      const int k = r >> (nsub - 1);
+      if (c < nC) dst[(size_t)r * a.out_is + c] = buf[k * nl + s * C + c];
+    }
+    return;
+  }
+
+  if (MODE == COL_CONV) {
+    int fy = o * a.gym + a.gya;
+    const int ry = min(fy, a.Y2 - fy);
+    const int tot = nsub * L * C;
+    for (int e = threadIdx.x; e < tot; e += NT) {
+      const int c = e & (C - 1);
+      const int r = e >> lgC;         // r = k*nsub + s
+      const int s = r & (nsub - 1);
+      const int k = r >> (nsub - 1);
+      const int fz = 2 * k + (nsub == 2 ? s : a.zpar);
+      const int rz = min(fz, a.Z2 - fz);
+      int fx = (col0 + min(c, nC - 1)) * a.gxm + a.gxa;
+      const int rx = min(fx, a.X2 - fx);
+      const T g = G[((size_t)rz * a.GY1 + ry) * a.GX1 + rx];
+      const int idx = k * nl + s * C + c;
+      buf[idx] = cscale<T>(buf[idx], g);
+    }
+    __syncthreads();
+  }
+
+  lds_fft<true, EMAX, false, T>(buf, tw, p, nl, lgnl);
+  {
+    const int tot = L * C;
+    for (int e = threadIdx.x; e < tot; e += NT) {
+      const int c = e & (C - 1);
+      const int n = e >> lgC;
+      if (c >= nC) continue;
+      C2<T> v = buf[n * nl + c];
+      if (nsub == 2) v = cadd<T>(v, cmulc<T>(buf[n * nl + C + c], h[n]));
+      dst[(size_t)n * a.out_is + c] = v;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Interaction table (FP64) -- InteractionBase::map (Interaction.hpp:32-51)
+// ---------------------------------------------------------------------------
+
+struct GArgs {
+  int X1, Y1, Z1;      // table extent (Xi+1, Yi+1, Zi+1)
+  int ux, uy, uz;      // extent of the user-visible octant (X+1, ...); beyond it G = 0
+  double sx, sy, sz;   // grid steps
+  double onsite;
+  double dielectric;
+  int tf;              // Thomas-Fermi-Resta model
+  double rTF, qTF;
+};
+
+__device__ __forceinline__ double eps_of_r(const GArgs& g, double r) {
+  // InteractionThomasFermi::dielectric (Interaction.cpp:34-39)
+  if (g.tf && r < g.rTF) return g.dielectric * (g.qTF * g.rTF / (sinh(g.qTF * (g.rTF - r)) + g.qTF * r));
+  return g.dielectric;
+}
+
+__global__ void k_gtable(double* __restrict__ G, GArgs g) {
+  const size_t n = (size_t)g.X1 * g.Y1 * g.Z1;
+  for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < n; t += (size_t)gridDim.x * blockDim.x) {
+    const int ix = (int)(t % g.X1);
+    const size_t q = t / g.X1;
+    const int iy = (int)(q % g.Y1);
+    const int iz = (int)(q / g.Y1);
+    double v = 0.0;
+    if (ix < g.ux && iy < g.uy && iz < g.uz) {
+      const double z = iz * g.sz, z2 = z * z;
+      const double y = iy * g.sy, y2z2 = z2 + y * y;
+      const double x = ix * g.sx, r2 = y2z2 + x * x, r = sqrt(r2);
+      v = 14.39963737103201 / (eps_of_r(g, r) * r);
+      if (ix == 0 && iy == 0 && iz == 0) v = g.onsite;
+    }
+    G[t] = v;
+  }
+}
+
+// Pads a user table of (ux)(uy)(uz) into the (X1)(Y1)(Z1) engine table.
+__global__ void k_gpad(const double* __restrict__ src, double* __restrict__ G, GArgs g) {
+  const size_t n = (size_t)g.X1 * g.Y1 * g.Z1;
+  for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < n; t += (size_t)gridDim.x * blockDim.x) {
+    const int ix = (int)(t % g.X1);
+    const size_t q = t / g.X1;
+    const int iy = (int)(q % g.Y1);
+    const int iz = (int)(q / g.Y1);
+    G[t] = (ix < g.ux && iy < g.uy && iz < g.uz) ? src[((size_t)iz * g.uy + iy) * g.ux + ix] : 0.0;
+  }
+}
+
+// DCT-I (FFTW REDFT00, unnormalised) of every line of the table along one
+// axis, through the 2M-point FFT of the even extension (M = extent - 1):
+//   Y_k = g_0 + (-1)^k g_M + 2 sum_{n=1}^{M-1} g_n cos(pi n k / M),  k = 0..M.
+// Line element i of line (o, c) is at G[o*os + i*is + c*cs]; a block holds
+// C lines.  The tile is index-major when cs == 1 (consecutive columns) and
+// line-major when is == 1.  The result is scaled by `scale`.
+template <int EMAX, bool LM>
+__global__ __launch_bounds__(NT) void k_dct1(double* __restrict__ G, AxisPlan p,
+                                             const double2* __restrict__ tw, int nlines, int C, int lgC,
+                                             long long os, long long is, long long cs, int ncols,
+                                             double scale) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  double2* buf = reinterpret_cast<double2*>(smem_raw);
+  const int L2 = p.L;      // 2M
+  const int M = L2 / 2;
+  // block -> (outer o, first column c0)
+  const int ncb = (ncols + C - 1) / C;
+  const int o = blockIdx.x / ncb;
+  const int c0 = (blockIdx.x - o * ncb) * C;
+  const int nC = min(C, ncols - c0);
+  double* base = G + (size_t)o * os + (size_t)c0 * cs;
+  const int tot = C * L2;
+  for (int e = threadIdx.x; e < tot; e += NT) {
+    int c, m;
+    if (LM) { c = (int)fdiv((uint32_t)e, p.l_magic); m = e - c * L2; }
+    else { c = e & (C - 1); m = e >> lgC; }
+    const int i = m <= M ? m : L2 - m;
+    double v = 0.0;
+    if (c < nC) v = base[(size_t)i * is + (size_t)c * cs];
+    buf[LM ? c * L2 + m : m * C + c] = make_double2(v, 0.0);
+  }
+  __syncthreads();
+  lds_fft<false, EMAX, LM, double>(buf, tw, p, C, lgC);
+  const int tot1 = C * (M + 1);
+  for (int e = threadIdx.x; e < tot1; e += NT) {
+    int c, k;
+    if (LM) { c = e / (M + 1); k = e - c * (M + 1); }
+    else { c = e & (C - 1); k = e >> lgC; }
+    if (c < nC) base[(size_t)k * is + (size_t)c * cs] = buf[LM ? c * L2 + k : k * C + c].x * scale;
+  }
+  (void)nlines;
+}
+
+// ---------------------------------------------------------------------------
+// Small elementwise kernels
+// ---------------------------------------------------------------------------
+
+template <typename T>
+__global__ void k_zero(C2<T>* __restrict__ p, size_t n) {
+  for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < n; t += (size_t)gridDim.x * blockDim.x)
+    p[t] = mk<T>((T)0, (T)0);
+}
+
+__global__ void k_d2f(const double* __restrict__ a, float* __restrict__ b, size_t n) {
+  for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < n; t += (size_t)gridDim.x * blockDim.x)
+    b[t] = (float)a[t];
+}
+
+// Engine grid (Xi,Yi,Zi) <-> user grid (X,Y,Z) copies with precision change.
+template <typename T>
+__global__ void k_grid_in(const double2* __restrict__ src, C2<T>* __restrict__ dst, int X, int Y, int Z,
+                          int Xi, int Yi) {
+  const size_t n = (size_t)X * Y * Z;
+  for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < n; t += (size_t)gridDim.x * blockDim.x) {
+    const int ix = (int)(t % X);
+    const size_t q = t / X;
+    const int iy = (int)(q % Y);
+    const int iz = (int)(q / Y);
+    const double2 v = src[t];
+    dst[((size_t)iz * Yi + iy) * Xi + ix] = mk<T>((T)v.x, (T)v.y);
+  }
+}
+
+template <typename T>
+__global__ void k_grid_out(const C2<T>* __restrict__ src, double2* __restrict__ dst, int X, int Y, int Z,
+                           int Xi, int Yi) {
+  const size_t n = (size_t)X * Y * Z;
+  for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < n; t += (size_t)gridDim.x * blockDim.x) {
+    const int ix = (int)(t % X);
+    const size_t q = t / X;
+    const int iy = (int)(q % Y);
+    const int iz = (int)(q / Y);
+    const C2<T> v = src[((size_t)iz * Yi + iy) * Xi + ix];
+    dst[t] = make_double2((double)v.x, (double)v.y);
+  }
+}
+
+// F[cell[k]] = rho[k]  (Product::map, Product.cpp:16-30; the grid is zeroed first)
+template <typename T>
+__global__ void k_scatter(C2<T>* __restrict__ F, const int32_t* __restrict__ cell,
+                          const double2* __restrict__ rho, int64_t n) {
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x) {
+    const double2 v = rho[t];
+    F[cell[t]] = mk<T>((T)v.x, (T)v.y);
+  }
+}
+
+// out[k] = V[cell[k]] in FP64
+template <typename T>
+__global__ void k_gather(const C2<T>* __restrict__ V, const int32_t* __restrict__ cell,
+                         double2* __restrict__ out, int64_t n) {
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x) {
+    const C2<T> v = V[cell[t]];
+    out[t] = make_double2((double)v.x, (double)v.y);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Pair densities: rho_P[k] = sum_o conj(c_L[a,o]) c_R[a,o] for every product
+// P = L(L+1)/2 + R, R <= L (FunctionCollection::createProducts,
+// FunctionCollection.cpp:91-105; ProductFromTightBinding::map, Product.cpp:16-30).
+// coeffs: [state][atom][orbital]; atom[k]: atom of effective site k.
+// A block stages A sites x all states in LDS.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(NT) void k_products(const double2* __restrict__ coeffs,
+                                                 const int32_t* __restrict__ atom, int64_t n_sites,
+                                                 int64_t n_atoms, int n_states, int n_orb, int A,
+                                                 double2* __restrict__ rho) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  double2* cs = reinterpret_cast<double2*>(smem_raw);  // [s][a][o]
+  const int64_t k0 = (int64_t)blockIdx.x * A;
+  const int nA = (int)min((int64_t)A, n_sites - k0);
+  const int per = A * n_orb;
+  const int tot = n_states * per;
+  for (int e = threadIdx.x; e < tot; e += NT) {
+    const int s = e / per;
+    const int r = e - s * per;
+    const int a = r / n_orb;
+    const int o = r - a * n_orb;
+    double2 v = make_double2(0.0, 0.0);
+    if (a < nA) v = coeffs[((size_t)s * n_atoms + atom[k0 + a]) * n_orb + o];
+    cs[e] = v;
+  }
+  __syncthreads();
+  const int P = n_states * (n_states + 1) / 2;
+  const int tot2 = P * A;
+  for (int e = threadIdx.x; e < tot2; e += NT) {
+    const int pidx = e / A;
+    const int a = e - pidx * A;
+    if (a >= nA) continue;
+    // pidx -> (fL, fR) with pidx = fL(fL+1)/2 + fR
+    int fL = (int)((sqrt(8.0 * pidx + 1.0) - 1.0) * 0.5);
+    while (fL * (fL + 1) / 2 > pidx) --fL;
+    while ((fL + 1) * (fL + 2) / 2 <= pidx) ++fL;
+    const int fR = pidx - fL * (fL + 1) / 2;
+    const double2* l = cs + (size_t)fL * per + a * n_orb;
+    const double2* r = cs + (size_t)fR * per + a * n_orb;
+    double re = 0.0, im = 0.0;
+    for (int o = 0; o < n_orb; ++o) {
+      const double2 x = l[o], y = r[o];
+      re += x.x * y.x + x.y * y.y;
+      im += x.x * y.y - x.y * y.x;
+    }
+    rho[(size_t)pidx * n_sites + k0 + a] = make_double2(re, im);
+  }
+}
+
+// Same without LDS staging, for state counts whose staging would not fit.
+__global__ void k_products_direct(const double2* __restrict__ coeffs, const int32_t* __restrict__ atom,
+                                  int64_t n_sites, int64_t n_atoms, int n_states, int n_orb,
+                                  double2* __restrict__ rho) {
+  const int P = n_states * (n_states + 1) / 2;
+  const int64_t tot = (int64_t)P * n_sites;
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < tot; t += (int64_t)gridDim.x * blockDim.x) {
+    const int pidx = (int)(t / n_sites);
+    const int64_t k = t - (int64_t)pidx * n_sites;
+    int fL = (int)((sqrt(8.0 * pidx + 1.0) - 1.0) * 0.5);
+    while (fL * (fL + 1) / 2 > pidx) --fL;
+    while ((fL + 1) * (fL + 2) / 2 <= pidx) ++fL;
+    const int fR = pidx - fL * (fL + 1) / 2;
+    const double2* l = coeffs + ((size_t)fL * n_atoms + atom[k]) * n_orb;
+    const double2* r = coeffs + ((size_t)fR * n_atoms + atom[k]) * n_orb;
+    double re = 0.0, im = 0.0;
+    for (int o = 0; o < n_orb; ++o) {
+      const double2 x = l[o], y = r[o];
+      re += x.x * y.x + x.y * y.y;
+      im += x.x * y.y - x.y * y.x;
+    }
+    rho[t] = make_double2(re, im);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Overlaps (CoulombCalculator::calculate, CoulombCalculator.cpp:165-172) at
+// atom sites.  For every (left slot l, right product r) four real sums over
+// the sites are formed,
+//   A = sum Vr*pr, B = sum Vi*pi, C = sum Vr*pi, D = sum Vi*pr,
+// from which  sum V*rho = (A-B) + i(C+D)  and  sum V*conj(rho) = (A+B) + i(D-C).
+// Stage 1: a 48x48 (slot, product) tile per block, 3x3 pairs per thread, over
+// one chunk of sites; partial sums to part[chunk][l][r][4].
+// ---------------------------------------------------------------------------
+constexpr int OV_T = 48;   // tile edge
+constexpr int OV_S = 32;   // sites staged per step
+
+__global__ __launch_bounds__(NT) void k_overlap_partial(const double2* __restrict__ Vat,
+                                                        const double2* __restrict__ rho,
+                                                        int n_left, int n_right, int64_t n_sites,
+                                                        int64_t chunk, double* __restrict__ part) {
+  __shared__ double2 sv[OV_T][OV_S + 1];
+  __shared__ double2 sr[OV_T][OV_S + 1];
+  const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
+  const int l0 = blockIdx.x * OV_T, r0 = blockIdx.y * OV_T;
+  const int64_t s0 = (int64_t)blockIdx.z * chunk;
+  const int64_t s1 = min(n_sites, s0 + chunk);
+  double acc[3][3][4];
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j)
+#pragma unroll
+      for (int q = 0; q < 4; ++q) acc[i][j][q] = 0.0;
+  for (int64_t sb = s0; sb < s1; sb += OV_S) {
+    for (int e = threadIdx.x; e < OV_T * OV_S; e += NT) {
+      const int row = e / OV_S, col = e - row * OV_S;
+      const int64_t site = sb + col;
+      const bool ok = site < s1;
+      sv[row][col] = (ok && l0 + row < n_left) ? Vat[(size_t)(l0 + row) * n_sites + site] : make_double2(0.0, 0.0);
+      sr[row][col] = (ok && r0 + row < n_right) ? rho[(size_t)(r0 + row) * n_sites + site] : make_double2(0.0, 0.0);
+    }
+    __syncthreads();
+#pragma unroll 4
+    for (int s = 0; s < OV_S; ++s) {
+      double2 v[3], r[3];
+#pragma unroll
+      for (int i = 0; i < 3; ++i) { v[i] = sv[ty + 16 * i][s]; r[i] = sr[tx + 16 * i][s]; }
+#pragma unroll
+      for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+          acc[i][j][0] = fma(v[i].x, r[j].x, acc[i][j][0]);
+          acc[i][j][1] = fma(v[i].y, r[j].y, acc[i][j][1]);
+          acc[i][j][2] = fma(v[i].x, r[j].y, acc[i][j][2]);
+          acc[i][j][3] = fma(v[i].y, r[j].x, acc[i][j][3]);
+        }
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      const int l = l0 + ty + 16 * i, r = r0 + tx + 16 * j;
+      if (l < n_left && r < n_right) {
+        double* d = part + (((size_t)blockIdx.z * n_left + l) * n_right + r) * 4;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) d[q] = acc[i][j][q];
+      }
+    }
+}
+
+// Stage 2: compensated (Neumaier) sum of the chunk partials.
+__global__ void k_overlap_reduce(const double* __restrict__ part, int64_t n_chunks, int64_t n_out,
+                                 double* __restrict__ out) {
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n_out; t += (int64_t)gridDim.x * blockDim.x) {
+    double s = 0.0, comp = 0.0;
+    for (int64_t c = 0; c < n_chunks; ++c) {
+      const double x = part[(size_t)c * n_out + t];
+      const double u = s + x;
+      comp += (fabs(s) >= fabs(x)) ? (s - u) + x : (x - u) + s;
+      s = u;
+    }
+    out[t] = s + comp;
+  }
+}
+
+// Full-grid bilinear dot sum_n V(n) F(n) (the reference's calculate()).
+// Per-thread FP64 sums, wave and block reduction, partials per block.
+template <typename T>
+__global__ __launch_bounds__(NT) void k_grid_dot(const C2<T>* __restrict__ V, const double2* __restrict__ F,
+                                                 int X, int Y, int Z, int Xi, int Yi,
+                                                 double* __restrict__ part) {
+  const size_t n = (size_t)X * Y * Z;
+  double re = 0.0, im = 0.0;
+  for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < n; t += (size_t)gridDim.x * blockDim.x) {
+    const double2 f = F[t];
+    if (f.x == 0.0 && f.y == 0.0) continue;
+    const int ix = (int)(t % X);
+    const size_t q = t / X;
+    const int iy = (int)(q % Y);
+    const int iz = (int)(q / Y);
+    const C2<T> v = V[((size_t)iz * Yi + iy) * Xi + ix];
+    re += (double)v.x * f.x - (double)v.y * f.y;
+    im += (double)v.x * f.y + (double)v.y * f.x;
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    re += __shfl_xor(re, off, 64);
+    im += __shfl_xor(im, off, 64);
+  }
+  __shared__ double sre[NT / 64], sim[NT / 64];
+  const int w = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) { sre[w] = re; sim[w] = im; }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double a = 0.0, b = 0.0;
+    for (int i = 0; i < NT / 64; ++i) { a += sre[i]; b += sim[i]; }
+    part[2 * blockIdx.x] = a;
+    part[2 * blockIdx.x + 1] = b;
+  }
+}
+
+}  // namespace clat

@@ -1,0 +1,199 @@
+/*
+ * clat.h -- C ABI of the MI355X FFT Coulomb-integral engine.
+ *
+ * Every entry point returns an int status (CLAT_OK == 0) and never throws.
+ * On failure clat_last_error() returns a thread-local message.  Host arrays
+ * passed in are borrowed for the duration of the call only; the engine owns
+ * all device memory.  One host thread per engine handle.  All calls are
+ * ordered on the engine's own HIP stream and synchronous at return.
+ *
+ * Complex numbers are passed as interleaved (re, im) doubles.  Grids are
+ * x-fastest: cell (ix, iy, iz) lives at (iz*Y + iy)*X + ix, the layout of the
+ * reference's Domain views (src/Domain.hpp:32, run-tests.cpp:170-175).
+ *
+ * Reference interfaces each entry point replaces are cited as
+ * file:line relative to the reference repository root.
+ */
+#ifndef CLAT_H
+#define CLAT_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------ */
+#define CLAT_OK        0
+#define CLAT_EINVAL    1  /* bad argument (reference: std::runtime_error / logic_error) */
+#define CLAT_ENOMEM    2  /* device or host allocation failed */
+#define CLAT_EDEVICE   3  /* HIP runtime error or no usable GPU */
+#define CLAT_ESTATE    4  /* call out of protocol order (e.g. prepare before initialize) */
+#define CLAT_EUNSUP    5  /* size outside what the kernels support */
+
+/* ---- engine options ---------------------------------------------------- */
+/* Arithmetic of the FFT passes.  The interaction table, its DCT-I and every
+ * overlap sum are FP64 in both modes. */
+#define CLAT_PREC_FP64 0  /* complex128 passes, phases and kernel multiply */
+#define CLAT_PREC_FP32 1  /* complex64 passes, phases and kernel multiply  */
+
+/* Schedule of the eight half-shift rounds (CoulombCalculator.cpp:105-163). */
+#define CLAT_SCHED_SHARED 0  /* rounds share their x and y passes: the eight
+                                parity classes are produced by pruned 2X/2Y/2Z
+                                transforms of the unpadded grid              */
+#define CLAT_SCHED_ROUNDS 1  /* eight independent rounds, each a full forward
+                                and inverse 3-D FFT, as the reference loops   */
+
+/* flags of clat_compute_integrals */
+#define CLAT_FLAG_STALE_VALUES 1  /* reproduce coulombo.cpp:194-205 exactly:
+                                     reuse the previous overlap when the right
+                                     product is unchanged across a new left */
+
+typedef struct clat_engine clat_engine;
+
+/* Thread-local text of the last failure ("" if none). */
+const char* clat_last_error(void);
+
+/* Library version string, e.g. "clat 1.0 (gfx950)". */
+const char* clat_version(void);
+
+/* ---- host-side lattice and planning helpers (no GPU needed) ------------ */
+
+/* Round::up<p1,p2,...>(n, c): smallest integer >= n of the form
+ * c * p1^a * p2^b * ...   Replaces src/Round.hpp:20-27,38-49. */
+int clat_round_up(int n, int c, const int* primes, int n_primes, int* out);
+
+/* Lattice detection of src/Broadcaster.cpp:24-43,53-56 (computeGrid x3 +
+ * round4FFT).  xyz is atom-major (x,y,z per atom, Angstrom).  Outputs raw and
+ * 2*3*5-padded sizes, origin and step per axis.  Fails with CLAT_EINVAL and
+ * "could not find common grid step" as Broadcaster.cpp:38. */
+int clat_lattice_grid(const double* xyz, int64_t n_atoms, int dim_raw[3],
+                      int dim_pad[3], double origin[3], double step[3]);
+
+/* Atom -> linear cell index ((iz*Y + iy)*X + ix) with lrint rounding,
+ * Broadcaster.cpp:76-88 at one rank.  dim is the padded dimension. */
+int clat_lattice_cells(const double* xyz, int64_t n_atoms, const int dim[3],
+                       const double origin[3], const double step[3],
+                       int32_t* cell);
+
+/* Enumerate, in the order of coulombo.cpp:135-148, every (i1,i2,i3,i4) in
+ * [1,n_states]^4 matched by the comma-separated pattern list (src/Pattern.hpp).
+ * Writes up to cap quads (4 ints each, 1-based) and the total count.
+ * quads may be NULL to count only. */
+int clat_pattern_enumerate(const char* spec, int n_states, int n_holes,
+                           int32_t* quads, int64_t cap, int64_t* count);
+
+/* Greedy vertex cover of src/Graph.hpp:49-91.  edges: n_edges (u,v) pairs.
+ * cover receives the sorted cover (capacity n_vertices). */
+int clat_vertex_cover(int n_vertices, const int32_t* edges, int64_t n_edges,
+                      int32_t* cover, int* n_cover);
+
+/* MasterPlanner (src/Planner.hpp:65-145): for each quad, left = product(i1,i4),
+ * right = product(i2,i3); cover swap; sort.  steps receives n rows of
+ * {id, left, left_conj, right, right_conj} in execution order. */
+int clat_plan(const int32_t* quads, int64_t n, int n_states, int32_t* steps);
+
+/* ---- engine: the CoulombCalculator protocol --------------------------- */
+
+/* CoulombCalculator(const Dimension&, bool) -- CoulombCalculator.cpp:11-55.
+ * X, Y, Z: grid dimension.  precision: CLAT_PREC_*, schedule: CLAT_SCHED_*.
+ * Sizes with a prime factor above 7 are carried on the next 2*3*5*7-smooth
+ * grid; results are unchanged because the open-boundary convolution is exact
+ * for any padding. */
+int clat_create(int X, int Y, int Z, int precision, int schedule,
+                clat_engine** out);
+
+/* Releases every device buffer of the engine. */
+int clat_destroy(clat_engine* e);
+
+/* InteractionSimple + InteractionBase::map + CoulombCalculator::initialize
+ * (Interaction.hpp:32-51, Interaction.cpp:8-14, CoulombCalculator.cpp:57-103):
+ * builds G(ix,iy,iz) = 14.39963737103201/(dielectric*r) on the
+ * (X+1)(Y+1)(Z+1) first octant with G(0,0,0) = onsite, then its 3-D DCT-I
+ * divided by 8*X*Y*Z -- all in FP64 on the GPU. */
+int clat_set_interaction_simple(clat_engine* e, const double step[3],
+                                double onsite, double dielectric);
+
+/* InteractionThomasFermi (Interaction.cpp:18-39), dielectric > 1,
+ * lattice_constant > 0 (coulombo.cpp:166-174). */
+int clat_set_interaction_thomas_fermi(clat_engine* e, const double step[3],
+                                      double onsite, double dielectric,
+                                      double lattice_constant);
+
+/* Arbitrary Interaction::map result (Interaction.hpp:14): G on the
+ * (X+1)(Y+1)(Z+1) first octant, x fastest, then CoulombCalculator::initialize. */
+int clat_set_interaction_table(clat_engine* e, const double* G);
+
+/* Write the calculator's `input` domain (CoulombCalculator.hpp:48): F is
+ * X*Y*Z complex numbers. */
+int clat_set_input(clat_engine* e, const double* F);
+
+/* CoulombCalculator::prepare (CoulombCalculator.cpp:105-163): V = G * input. */
+int clat_prepare(clat_engine* e);
+
+/* CoulombCalculator::calculate (CoulombCalculator.cpp:165-172):
+ * sum_n V(n) * input(n), bilinear, FP64 compensated sum. */
+int clat_calculate(clat_engine* e, double result[2]);
+
+/* CoulombCalculator::potential (CoulombCalculator.hpp:90-93): copies V
+ * (X*Y*Z complex) to the host. */
+int clat_get_potential(clat_engine* e, double* V);
+
+/* ---- engine: atom-site path (FunctionCollection + Product + main loop) - */
+
+/* Atom -> cell map of the lattice (Broadcaster.cpp:76-88) and the orbital
+ * count (coulombo.cpp:59-65).  Where several atoms share a cell, the last
+ * one wins, as Product::map's assignment does (Product.cpp:22,27). */
+int clat_set_atoms(clat_engine* e, const int32_t* cell, int64_t n_atoms,
+                   int n_orbitals);
+
+/* FunctionCollection::appendFile (FunctionCollection.cpp:56-88): one state,
+ * n_atoms*n_orbitals complex coefficients, atom-major / orbital-minor.
+ * States are numbered 1.. in call order; holes must come first. */
+int clat_add_state(clat_engine* e, const double* coeffs);
+
+/* Drop every state added so far. */
+int clat_clear_states(clat_engine* e);
+
+/* The coulombo main loop (coulombo.cpp:150-209) for the given quads
+ * (n rows of 1-based i1,i2,i3,i4): plan, pair densities, prepares of the
+ * cover, overlaps, conjugation bookkeeping.  out: n complex values in quad
+ * order.  flags: CLAT_FLAG_*. */
+int clat_compute_integrals(clat_engine* e, const int32_t* quads, int64_t n,
+                           int flags, double* out);
+
+/* potentials.cpp:140-158: V of the self product of `state` (1-based) at every
+ * atom's cell (FunctionCollection::extractAtomCellValues,
+ * FunctionCollection.cpp:122-131).  v: n_atoms complex values. */
+int clat_compute_potential_at_atoms(clat_engine* e, int state, double* v);
+
+/* ---- measurement ------------------------------------------------------- */
+
+/* Stage times in milliseconds from HIP events on the engine stream:
+ * [0] the last interaction build (G table + DCT-I + scaling), and for the
+ * last clat_compute_integrals call [1] pair densities, [2] all prepares,
+ * [3] overlaps, [4] the whole call. */
+int clat_last_stage_ms(clat_engine* e, double ms[5]);
+
+/* When enabled, every FFT pass launch is bracketed by HIP events on the
+ * engine stream and timed per kernel kind. */
+int clat_set_kernel_timing(clat_engine* e, int enable);
+
+/* Per-kind totals since the last reset: launches, summed milliseconds and the
+ * algorithmic bytes those launches must move (one read of each input element
+ * and one write of each output element).  kind < clat_kernel_kind_count(). */
+int clat_kernel_kind_count(void);
+const char* clat_kernel_kind_name(int kind);
+int clat_kernel_stats(clat_engine* e, int kind, int64_t* launches,
+                      double* total_ms, double* bytes);
+int clat_reset_kernel_stats(clat_engine* e);
+
+/* Grid size the engine computes on (after 2*3*5*7 padding) and the device
+ * bytes it currently holds. */
+int clat_engine_info(clat_engine* e, int dim[3], int64_t* device_bytes);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CLAT_H */

@@ -1,0 +1,158 @@
+"""GPU parity of the HIP engine against the FP64 CPU oracle and the
+reference's own known-answer values.  Every call goes through the C ABI."""
+import os
+
+import numpy as np
+import pytest
+
+import clat
+import oracle as O
+import zincblende as zb
+
+pytestmark = pytest.mark.gpu
+
+COMBOS = [(clat.PREC_FP64, clat.SCHED_SHARED), (clat.PREC_FP64, clat.SCHED_ROUNDS),
+          (clat.PREC_FP32, clat.SCHED_SHARED), (clat.PREC_FP32, clat.SCHED_ROUNDS)]
+IDS = ["fp64-shared", "fp64-rounds", "fp32-shared", "fp32-rounds"]
+# relative tolerance per precision: FP64 path 1e-10 (vs oracle), FP32 path 1e-5
+TOL = {clat.PREC_FP64: 1e-10, clat.PREC_FP32: 1e-5}
+
+
+def _mock(G):
+    # CoulombCalculatorTest_Interaction (run-tests.cpp:13-21): G = exp(-ix-iy-iz)
+    Z1, Y1, X1 = G.shape
+    iz, iy, ix = np.meshgrid(np.arange(Z1), np.arange(Y1), np.arange(X1), indexing="ij")
+    G[...] = np.exp(-ix - iy - iz)
+
+
+@pytest.mark.parametrize("prec,sched", COMBOS, ids=IDS)
+def test_two_point_charges(prec, sched):
+    """run-tests.cpp:28-48: 5x5x10 grid, delta at (2,2,2) -> 1.0; at (2,2,7) -> e^-5."""
+    print("case two_point_charges", prec, sched, flush=True)
+    cc = clat.CoulombCalculator((5, 5, 10), prec, sched)
+    cc.initialize(clat.InteractionTable(_mock))
+    tol = 1e-12 if prec == clat.PREC_FP64 else 1e-6
+    cc.input[...] = 0
+    cc.input[2, 2, 2] = 1.0
+    cc.prepare()
+    r = cc.calculate()
+    assert abs(r.real - 1.0) < tol and abs(r.imag) < tol
+    cc.input[...] = 0
+    cc.input[7, 2, 2] = 1.0
+    r = cc.calculate()
+    assert abs(r.real - np.exp(-5)) < tol and abs(r.imag) < tol
+
+
+@pytest.mark.parametrize("prec,sched", COMBOS, ids=IDS)
+@pytest.mark.parametrize("dims", [(6, 5, 7), (16, 8, 4), (9, 10, 12), (11, 13, 5), (1, 4, 3)])
+def test_prepare_matches_oracle(prec, sched, dims):
+    """Random complex input on odd, mixed-radix, non-smooth (11, 13) and
+    degenerate grids: potential and calculate() vs the 8-round oracle."""
+    X, Y, Z = dims
+    print("case prepare", dims, prec, sched, flush=True)
+    rng = np.random.default_rng(X * 10000 + Y * 100 + Z)
+    F = rng.standard_normal((Z, Y, X)) + 1j * rng.standard_normal((Z, Y, X))
+    step = (1.1, 0.9, 1.3)
+    Gt = O.gtable(X, Y, Z, step, 0.37, 3.0)
+    Vo = O.prepare(F, O.ghat(Gt))
+    cc = clat.CoulombCalculator(dims, prec, sched)
+    cc.initialize(clat.InteractionSimple(step, 0.37, 3.0))
+    cc.input[...] = F
+    cc.prepare()
+    V = cc.potential()
+    err = np.max(np.abs(V - Vo)) / np.max(np.abs(Vo))
+    assert err < TOL[prec] * 0.1, err
+    r = cc.calculate()
+    ro = O.calculate(Vo, F)
+    assert abs(r - ro) / abs(ro) < TOL[prec], (r, ro)
+
+
+def _example():
+    d = os.path.join(os.path.dirname(__file__), "golden", "example")
+    xyz = np.loadtxt(os.path.join(d, "X.3d"), usecols=(0, 1, 2))
+    c = np.loadtxt(os.path.join(d, "e1.dat"), skiprows=1, usecols=(0, 1))
+    return xyz, (c[:, 0] + 1j * c[:, 1]).reshape(len(xyz), 10)
+
+
+@pytest.mark.parametrize("prec,sched", COMBOS, ids=IDS)
+def test_example_1111(prec, sched):
+    """example/run.sh: 8610-atom Si:P, 10 orbitals, --dielectric=11.4 ->
+    ' 1  1  1  1    0.090340071977' (first 29 characters of eeee.txt)."""
+    print("case example", prec, sched, flush=True)
+    xyz, c = _example()
+    raw, pad, origin, step = clat.lattice_grid(xyz)
+    assert list(pad) == [45, 45, 45]
+    cells = clat.lattice_cells(xyz, pad, origin, step)
+    eng = clat.Engine(pad, prec, sched)
+    eng.initialize(clat.InteractionSimple(step, 0.0, 11.4))
+    eng.set_atoms(cells, 10)
+    eng.add_state(c)
+    v = eng.compute_integrals([(1, 1, 1, 1)])[0]
+    line = "%2d %2d %2d %2d   %17.14f %17.14f\n" % (1, 1, 1, 1, v.real, v.imag)
+    direct = 0.09034007197752  # FP64 direct lattice sum, pinned by the CPU tests
+    assert abs(v.real - direct) / direct < TOL[prec]
+    if prec == clat.PREC_FP64:
+        assert line[:29] == " 1  1  1  1    0.090340071977"
+
+
+@pytest.mark.parametrize("prec,sched", COMBOS, ids=IDS)
+def test_zincblende_integrals(prec, sched):
+    """2x2x2-cell zincblende (64 atoms, 8^3 grid), 2 holes + 2 electrons,
+    20 orbitals, every integral (pattern ****) vs the oracle main loop."""
+    print("case zincblende", prec, sched, flush=True)
+    w = zb.workload(2, 2, 2, 2, 2, config=0, sigma_frac=0.2)
+    X, Y, Z = w["dims"]
+    quads = clat.pattern_enumerate("****", 4, 2)
+    eng = clat.Engine((X, Y, Z), prec, sched)
+    eng.initialize(clat.InteractionSimple(w["step"], 0.0, 10.89))
+    eng.set_atoms(w["cells"], 20)
+    for s in w["states"]:
+        eng.add_state(s)
+    got = eng.compute_integrals(quads)
+    rho = O.products(np.stack(w["states"]))
+    Gh = O.ghat(O.gtable(X, Y, Z, w["step"], 0.0, 10.89))
+    want = O.compute_integrals([tuple(q) for q in quads], 4, rho, w["cells"], (X, Y, Z), Gh)
+    scale = np.max(np.abs(want))
+    err = np.abs(got - want) / np.maximum(np.abs(want), 1e-3 * scale)
+    assert np.max(err) < TOL[prec], np.max(err)
+
+
+@pytest.mark.parametrize("prec", [clat.PREC_FP64, clat.PREC_FP32], ids=["fp64", "fp32"])
+def test_potential_at_atoms(prec):
+    """potentials.cpp:140-158: V of each self product at every atom's cell."""
+    print("case potentials", prec, flush=True)
+    w = zb.workload(2, 2, 3, 1, 1, config=0, sigma_frac=0.2, n_orb=6)
+    X, Y, Z = w["dims"]
+    eng = clat.Engine((X, Y, Z), prec, clat.SCHED_SHARED)
+    eng.initialize(clat.InteractionSimple(w["step"], 0.5, 4.0))
+    eng.set_atoms(w["cells"], 6)
+    for s in w["states"]:
+        eng.add_state(s)
+    Gh = O.ghat(O.gtable(X, Y, Z, w["step"], 0.5, 4.0))
+    for k, s in enumerate(w["states"]):
+        v = eng.potential_at_atoms(k + 1)
+        F = np.zeros(X * Y * Z, complex)
+        F[w["cells"]] = np.sum(np.abs(s) ** 2, axis=1)
+        Vo = O.prepare(F.reshape(Z, Y, X), Gh).reshape(-1)[w["cells"]]
+        assert np.max(np.abs(v - Vo)) / np.max(np.abs(Vo)) < TOL[prec]
+
+
+@pytest.mark.parametrize("prec", [clat.PREC_FP64, clat.PREC_FP32], ids=["fp64", "fp32"])
+def test_thomas_fermi(prec):
+    """InteractionThomasFermi (Interaction.cpp:18-39) table vs the oracle's."""
+    print("case thomas_fermi", prec, flush=True)
+    w = zb.workload(2, 2, 2, 1, 1, config=0, sigma_frac=0.2, n_orb=4)
+    X, Y, Z = w["dims"]
+    quads = clat.pattern_enumerate("****", 2, 1)
+    eng = clat.Engine((X, Y, Z), prec, clat.SCHED_SHARED)
+    eng.initialize(clat.InteractionThomasFermi(w["step"], 0.0, 10.89, 5.6533))
+    eng.set_atoms(w["cells"], 4)
+    for s in w["states"]:
+        eng.add_state(s)
+    got = eng.compute_integrals(quads)
+    rho = O.products(np.stack(w["states"]))
+    Gh = O.ghat(O.gtable(X, Y, Z, w["step"], 0.0, 10.89, tf_lattice=5.6533))
+    want = O.compute_integrals([tuple(q) for q in quads], 2, rho, w["cells"], (X, Y, Z), Gh)
+    scale = np.max(np.abs(want))
+    err = np.abs(got - want) / np.maximum(np.abs(want), 1e-3 * scale)
+    assert np.max(err) < TOL[prec]
