@@ -8,8 +8,9 @@ box), --integrals=eeee,hhhh,ehhe,eheh (1024 integrals), --dielectric=10.89.
 One step = the compute stage of one coulombo run with the states already in
 HBM: interaction table + DCT-I, pair densities, the 36 prepares, every
 overlap, the results back on the host.  value = integrals / second of the
-complex64-pass + FP64-accumulate path (median step).  The FP64 path runs the
-same inputs for the accuracy report (max relative error of every integral).
+complex64-storage / FP64-arithmetic path (CLAT_PREC_FP32X).  The FP64 path,
+the plain FP64 path (eight independent rounds) and pure FP32 run the same
+inputs for the accuracy and speed comparison.
 
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--cells NX NY NZ]
 """
@@ -109,34 +110,43 @@ def main():
         eng.initialize(inter)
         return eng.compute_integrals(quads)
 
+    # headline: complex64 storage + FP64 arithmetic (CLAT_PREC_FP32X), shared rounds.
+    # Beside it, on the same inputs: the FP64 path (accuracy reference), the
+    # plain FP64 path (the reference's eight independent rounds) and pure FP32.
+    runs = [("fp32x", clat.PREC_FP32X, sched), ("fp64", clat.PREC_FP64, sched),
+            ("fp64_rounds", clat.PREC_FP64, clat.SCHED_ROUNDS), ("fp32", clat.PREC_FP32, sched)]
+    if args.no_fp64:
+        runs = runs[:1]
     res = {}
-    for name, prec in (("fp32", clat.PREC_FP32), ("fp64", clat.PREC_FP64)):
-        if name == "fp64" and args.no_fp64:
-            continue
-        eng = make(prec)
+    for name, prec, sch in runs:
+        eng = clat.Engine((X, Y, Z), prec, sch)
+        eng.set_atoms(w["cells"], 20)
+        for s in w["states"]:
+            eng.add_state(s)
         for _ in range(args.warmup):
             step(eng)
-        eng.set_kernel_timing(name == "fp32")
+        eng.set_kernel_timing(name == "fp32x")
         eng.reset_kernel_stats()
+        steps = args.steps if name in ("fp32x", "fp64") else max(1, args.steps // 3)
         times = []
         vals = None
         t_all = time.perf_counter()
-        for _ in range(args.steps):
+        for _ in range(steps):
             t = time.perf_counter()
             vals = step(eng)
             times.append(time.perf_counter() - t)
         t_all = time.perf_counter() - t_all
         res[name] = dict(times=times, vals=vals, stats=eng.kernel_stats(), stage=eng.stage_ms().tolist(),
-                         total=t_all)
-        log("%s: median step %.2f ms  stages(ms) table %.2f rho %.2f prepares %.2f overlaps %.2f"
-            % (name, 1e3 * float(np.median(times)), *res[name]["stage"][:4]))
+                         total=t_all, steps=steps)
+        log("%s: mean step %.2f ms  stages(ms) table %.2f rho %.2f prepares %.2f overlaps %.2f"
+            % (name, 1e3 * t_all / steps, *res[name]["stage"][:4]))
         eng.close()
 
-    fp32 = res["fp32"]
-    ms = 1e3 * fp32["total"] / args.steps
-    value = len(quads) / (fp32["total"] / args.steps)
+    head = res["fp32x"]
+    ms = 1e3 * head["total"] / args.steps
+    value = len(quads) / (head["total"] / args.steps)
     # roofline of the dominant FFT pass kernel, from HIP events around each launch
-    stats = fp32["stats"]
+    stats = head["stats"]
     dom = max(stats, key=lambda k: stats[k][1])
     n_l, t_ms, byts = stats[dom]
     avg_ms = t_ms / max(1, n_l)
@@ -145,7 +155,8 @@ def main():
         "metric": "integrals/s, 1M-atom grid, FP32+FP64-acc vs FP64 at 1e-5 rel; FFT HBM GB/s",
         "value": value, "unit": "integrals/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "c64 passes + f64 accumulate", "data": "synthetic (seeded zincblende, see zincblende.py)",
+        "dtype": "c64 storage, f64 arithmetic and accumulation",
+        "data": "synthetic (seeded zincblende, see zincblende.py)",
         "config": {"workload": "zincblende %dx%dx%d cells = %d atoms, grid %dx%dx%d, %d states, "
                                "eeee,hhhh,ehhe,eheh" % (nx, ny, nz, len(w["cells"]), X, Y, Z, nh + ne),
                    "integrals": len(quads), "schedule": args.schedule},
@@ -153,14 +164,20 @@ def main():
                      "frac": achieved / PEAK_HBM_GBS, "traffic": None, "avg_launch_ms": avg_ms,
                      "algorithmic_bytes_per_launch": byts / max(1, n_l)},
         "kernel_ms_per_step": {k: v[1] / args.steps for k, v in stats.items() if v[0]},
+        "fft_pass_gbs": {k: (v[2] / max(1e-12, v[1] * 1e-3)) / 1e9 for k, v in stats.items()
+                         if v[0] and k in ("xfwd", "yfwd", "zconv", "yinv", "xinv")},
     }
     if "fp64" in res:
         v64 = res["fp64"]["vals"]
-        v32 = fp32["vals"]
-        rel = np.abs(v32 - v64) / np.abs(v64)
-        out["fp64_ms_per_step"] = 1e3 * res["fp64"]["total"] / args.steps
-        out["speedup_vs_fp64"] = out["fp64_ms_per_step"] / ms
-        out["max_rel_err_fp32_vs_fp64"] = float(np.max(rel))
+        for name in ("fp32x", "fp32", "fp64_rounds"):
+            if name in res:
+                rel = np.abs(res[name]["vals"] - v64) / np.abs(v64)
+                out["max_rel_err_%s_vs_fp64" % name] = float(np.max(rel))
+                out["%s_ms_per_step" % name] = 1e3 * res[name]["total"] / res[name]["steps"]
+        out["fp64_ms_per_step"] = 1e3 * res["fp64"]["total"] / res["fp64"]["steps"]
+        out["speedup_vs_fp64_shared"] = out["fp64_ms_per_step"] / ms
+        if "fp64_rounds" in res:
+            out["speedup_vs_plain_fp64_rounds"] = out["fp64_rounds_ms_per_step"] / ms
     if not args.no_cpu_baseline:
         try:
             out["cpu_baseline"] = cpu_baseline(w, quads)

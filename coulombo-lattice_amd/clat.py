@@ -23,7 +23,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libclat.so")
 
-PREC_FP64, PREC_FP32 = 0, 1
+PREC_FP64, PREC_FP32, PREC_FP32X = 0, 1, 2
 SCHED_SHARED, SCHED_ROUNDS = 0, 1
 FLAG_STALE_VALUES = 1
 

@@ -13,6 +13,7 @@
 #include "../../include/clat.h"
 #include "host_logic.hpp"
 #include "kernels.hpp"
+#include "pass2.hpp"
 
 using namespace clat;
 
@@ -72,6 +73,7 @@ uint32_t magic_of(uint32_t d) {
   return d <= 1 ? 0u : (uint32_t)((((uint64_t)1 << 32) + d - 1) / d);
 }
 
+// Stockham plan (generic pass kernels and the DCT-I)
 bool make_plan(int L, AxisPlan& p) {
   std::memset(&p, 0, sizeof p);
   p.L = L;
@@ -98,7 +100,31 @@ bool make_plan(int L, AxisPlan& p) {
   return true;
 }
 
-// e^{-2 pi i m / (den)} for m in [0, n), from long double trigonometry
+// two-level split L = R1 * R2 (R1 >= R2, most balanced), false if none
+bool split2(int L, int& R1, int& R2) {
+  int best = -1;
+  for (int r2 = 1; r2 * r2 <= L; ++r2) {
+    if (L % r2) continue;
+    const int r1 = L / r2;
+    if (reg_size_ok(r1) && reg_size_ok(r2)) best = r2;
+  }
+  if (best < 0) return false;
+  R2 = best;
+  R1 = L / best;
+  return true;
+}
+
+// engine axis length for a user length: the smallest two-level size, else
+// the next 2*3*5*7-smooth size (generic kernels)
+int engine_len(int n) {
+  for (int m = n; m <= 400; ++m) {
+    int a, b;
+    if (split2(m, a, b)) return m;
+  }
+  return host::round2357(n);
+}
+
+// e^{-2 pi i m / den} for m in [0, n), from long double trigonometry
 std::vector<double> unit_roots(int n, int den) {
   std::vector<double> v(2 * (size_t)n);
   const long double pi = 3.141592653589793238462643383279502884L;
@@ -113,6 +139,11 @@ std::vector<double> unit_roots(int n, int den) {
 int floor_pow2(int v) {
   int p = 1;
   while (p * 2 <= v) p *= 2;
+  return p;
+}
+int ceil_pow2(int v) {
+  int p = 1;
+  while (p < v) p *= 2;
   return p;
 }
 int log2i(int v) {
@@ -133,6 +164,44 @@ unsigned grid1d(size_t n, int bs = 256) {
   return (unsigned)b;
 }
 
+// launch shape of one two-level pass
+struct PassCfg {
+  int R1 = 0, R2 = 0, R1p = 0, C = 1, lgC = 0, NT = 64;
+  size_t lds = 0;
+};
+
+int round64(int v) { return (v + 63) / 64 * 64; }
+
+PassCfg cfg_line(int L, int nsub, size_t c2bytes, bool sites, size_t budget) {
+  PassCfg p;
+  split2(L, p.R1, p.R2);
+  p.R1p = (p.R1 > 1 && p.R1 % 2 == 0) ? p.R1 + 1 : p.R1;
+  const size_t per = (size_t)std::max(nsub * p.R2 * p.R1p, L) * c2bytes;
+  int C = floor_pow2((int)std::max<size_t>(1, budget / per));
+  C = std::min(C, 64);
+  if (sites) C = std::min(C, std::max(1, floor_pow2(256 / p.R1)));
+  p.C = C;
+  p.lgC = log2i(C);
+  p.NT = std::min(256, std::max(64, round64(std::max(C * p.R1, C * nsub * p.R2))));
+  p.lds = (size_t)C * per;
+  return p;
+}
+
+PassCfg cfg_col(int L, int nsub, size_t c2bytes, int W, size_t budget) {
+  PassCfg p;
+  split2(L, p.R1, p.R2);
+  p.R1p = p.R1;
+  const size_t per = (size_t)nsub * L * c2bytes;
+  int C = floor_pow2((int)std::max<size_t>(1, budget / per));
+  C = std::min(C, c2bytes == 8 ? 32 : 16);
+  C = std::min(C, ceil_pow2(W));
+  p.C = C;
+  p.lgC = log2i(C);
+  p.NT = std::min(256, std::max(64, round64(std::max(C * p.R1, C * nsub * p.R2))));
+  p.lds = (size_t)C * per;
+  return p;
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -140,22 +209,29 @@ unsigned grid1d(size_t n, int bs = 256) {
 // ---------------------------------------------------------------------------
 struct clat_engine {
   int X = 0, Y = 0, Z = 0;     // user grid
-  int Xi = 0, Yi = 0, Zi = 0;  // engine grid (2*3*5*7-smooth)
-  int prec = 0, sched = 0;
+  int Xi = 0, Yi = 0, Zi = 0;  // engine grid
+  int prec = 0, sched = 0;     // requested precision and schedule
+  int rprec = 0;               // precision the passes run in
+  bool fast = false;           // two-level pass kernels on all three axes
   int device = 0;
   hipStream_t stream = nullptr;
-  AxisPlan px{}, py{}, pz{};   // line transforms
-  AxisPlan dx{}, dy{}, dz{};   // DCT-I via 2M-point transforms
-  size_t csz = 16;             // bytes per complex of the pass precision
-  int tile_budget = 0;         // max elements of one LDS tile
+  AxisPlan px{}, py{}, pz{};   // Stockham plans of the generic kernels
+  AxisPlan dx{}, dy{}, dz{};   // DCT-I through 2M-point transforms
+  size_t lds_budget = 65536;
+  int tile_budget = 0;         // generic kernels: elements of one LDS tile
+  PassCfg cxf, cyf, czc, cyi, cxi, cxf_s, cxi_s;  // two-level launch shapes (_s: atom sites)
+  // element sizes (bytes per complex) of the buffers
+  size_t s_in = 16, s_a = 16, s_b = 16, s_d = 16, s_v = 16;
   // device tables
-  void *tw[3] = {nullptr, nullptr, nullptr}, *half[3] = {nullptr, nullptr, nullptr};
-  double2* dtw[3] = {nullptr, nullptr, nullptr};
-  double* G64 = nullptr;  // Ghat, FP64
-  float* G32 = nullptr;   // Ghat, FP32 copy for the complex64 passes
+  void *tw32[3] = {}, *h32[3] = {}, *tw64[3] = {}, *h64[3] = {};
+  double2* dtw[3] = {};
+  double* G64 = nullptr;
+  float* G32 = nullptr;
   bool initialized = false;
   // grids
-  void *F = nullptr, *V = nullptr, *A = nullptr, *B = nullptr;
+  void *F = nullptr, *V = nullptr;  // dense input / potential (grid protocol, generic path)
+  void *A = nullptr, *B = nullptr, *D = nullptr;
+  size_t a_bytes = 0, b_bytes = 0, d_bytes = 0, f_bytes = 0, v_bytes = 0;
   double2* Fin = nullptr;  // user-layout FP64 copy of the input (calculate())
   bool input_set = false, prepared = false;
   double* dpart = nullptr;
@@ -163,14 +239,16 @@ struct clat_engine {
   // atoms and states
   int64_t n_atoms = 0;
   int n_orb = 0;
-  int32_t* d_cell = nullptr;       // all atoms
-  int64_t n_sites = 0;             // effective atoms (last atom of each cell)
-  int32_t* d_site_atom = nullptr;
-  int32_t* d_site_cell = nullptr;
+  int64_t n_sites = 0;               // effective atoms (last atom of each cell)
+  int32_t* d_site_atom = nullptr;    // atom of each site (sites ordered by cell)
+  int32_t* d_site_cell = nullptr;    // engine-grid cell of each site
+  int32_t* d_site_ptr = nullptr;     // CSR of sites over engine x-lines (Yi*Zi + 1)
+  int32_t* d_atom_site = nullptr;    // site holding each atom's cell
   int n_states = 0, cap_states = 0;
   double2* coeffs = nullptr;
   double2* rho = nullptr;
-  int rho_states = -1;             // states the pair densities were built for
+  size_t rho_bytes = 0;
+  int rho_states = -1;
   double2* Vat = nullptr;
   size_t vat_cap = 0;
   double* part = nullptr;
@@ -183,7 +261,7 @@ struct clat_engine {
   double stage_ms[5] = {0, 0, 0, 0, 0};
   bool ktiming = false;
   std::vector<hipEvent_t> kev;
-  std::vector<std::pair<int, double>> kpending;  // kind, bytes per pending launch
+  std::vector<std::pair<int, double>> kpending;
   size_t kused = 0;
   int64_t kcount[K_COUNT] = {};
   double kms[K_COUNT] = {};
@@ -195,9 +273,10 @@ struct clat_engine {
     device_bytes += (int64_t)bytes;
     return p;
   }
-  void dfree(void*& p, size_t bytes) {
+  template <typename P>
+  void dfree(P*& p, size_t bytes) {
     if (p) {
-      (void)hipFree(p);
+      (void)hipFree((void*)p);
       device_bytes -= (int64_t)bytes;
       p = nullptr;
     }
@@ -205,10 +284,8 @@ struct clat_engine {
   size_t N() const { return (size_t)Xi * Yi * Zi; }
   size_t NG() const { return (size_t)(Xi + 1) * (Yi + 1) * (Zi + 1); }
   size_t NU() const { return (size_t)X * Y * Z; }
-  size_t a_elems() const { return sched == CLAT_SCHED_SHARED ? 2 * N() : N(); }
-  size_t b_elems() const { return sched == CLAT_SCHED_SHARED ? 4 * N() : 0; }
+  bool shared() const { return sched == CLAT_SCHED_SHARED; }
 
-  // kernel timing around one launch
   void kbegin() {
     if (!ktiming) return;
     ensure_events(kused + 2);
@@ -245,27 +322,202 @@ struct clat_engine {
 
 namespace {
 
+// ===========================================================================
+// two-level pass launches
+// ===========================================================================
+template <typename TC>
+const C2<TC>* tw_of(const clat_engine* e, int axis) {
+  return (const C2<TC>*)(sizeof(TC) == 8 ? e->tw64[axis] : e->tw32[axis]);
+}
+template <typename TC>
+const C2<TC>* h_of(const clat_engine* e, int axis) {
+  return (const C2<TC>*)(sizeof(TC) == 8 ? e->h64[axis] : e->h32[axis]);
+}
+template <typename TC>
+const TC* g_of(const clat_engine* e) {
+  return (const TC*)(sizeof(TC) == 8 ? (const void*)e->G64 : (const void*)e->G32);
+}
+
+template <typename TI, typename TC, typename TO>
+constexpr int combo_of() {
+  constexpr bool di = sizeof(TI) == 8, dc = sizeof(TC) == 8, dout = sizeof(TO) == 8;
+  return (di && dc && dout) ? CB_DDD : (!di && !dc && !dout) ? CB_FFF : (di && !dc && !dout) ? CB_DFF
+       : (!di && !dc && dout) ? CB_FFD : (di && dc && !dout) ? CB_DDF : (!di && dc && !dout) ? CB_FDF
+       : CB_FDD;
+}
+
+template <typename TI, typename TC, typename TO, int MODE, bool LINE>
+void launch2(clat_engine* e, const PassCfg& pc, P2Args a, int n_outer, int axis, const void* in, void* out,
+             int kind, double bytes) {
+  a.R1 = pc.R1;
+  a.R2 = pc.R2;
+  a.R1p = pc.R1p;
+  a.mR1 = magic_of((uint32_t)pc.R1);
+  a.mR2 = magic_of((uint32_t)pc.R2);
+  a.mL = magic_of((uint32_t)a.L);
+  a.C = pc.C;
+  a.lgC = pc.lgC;
+  P2Launch L;
+  L.grid = dim3((unsigned)((a.W + pc.C - 1) / pc.C), (unsigned)n_outer);
+  L.block = dim3(pc.NT);
+  L.lds = pc.lds;
+  L.stream = e->stream;
+  L.in = in;
+  L.out = out;
+  L.a = a;
+  L.tw = tw_of<TC>(e, axis);
+  L.h = h_of<TC>(e, axis);
+  L.hy = h_of<TC>(e, 1);
+  L.hz = h_of<TC>(e, 2);
+  L.G = g_of<TC>(e);
+  constexpr int combo = combo_of<TI, TC, TO>();
+  e->kbegin();
+  hipError_t err;
+  switch (pc.R1 * 64 + pc.R2) {
+#define CLAT_P2_CASE(ra, rb) \
+  case ra * 64 + rb: err = clat_p2_##ra##x##rb(combo, MODE, LINE, L); break;
+    CLAT_P2_PAIRS(CLAT_P2_CASE)
+#undef CLAT_P2_CASE
+    default: err = p2_launch<0, 0>(combo, MODE, LINE, L); break;
+  }
+  HIPX(err);
+  e->kend(kind, bytes);
+}
+
+P2Args base_args(const clat_engine* e) {
+  P2Args a;
+  std::memset(&a, 0, sizeof a);
+  a.X2 = 2 * e->Xi;
+  a.Y2 = 2 * e->Yi;
+  a.Z2 = 2 * e->Zi;
+  a.GX1 = e->Xi + 1;
+  a.GY1 = e->Yi + 1;
+  a.Yl = e->Yi;
+  return a;
+}
+
+// One prepare() on the two-level kernels.
+//   TIN: dense input type, TA: x-forward output, TB: y-forward / conv output,
+//   TD: y-inverse output, TV: dense potential type, TC: arithmetic.
+// sites_in / sites_out: pair density at the sites and potential at the sites
+// (atom path); when null, the dense F / V grids are used.
+template <typename TIN, typename TA, typename TB, typename TD, typename TV, typename TC>
+void prepare_fast(clat_engine* e, const double2* sites_in, double2* sites_out) {
+  const int Xi = e->Xi, Yi = e->Yi, Zi = e->Zi;
+  const double N = (double)e->N();
+  const double sa = sizeof(C2<TA>), sb = sizeof(C2<TB>), sd = sizeof(C2<TD>), sv = sizeof(C2<TV>),
+               si = sizeof(C2<TIN>);
+  const double site_bytes = (double)e->n_sites * 20.0;  // value + cell index
+  const int nlines = Yi * Zi;
+  const bool atoms = sites_in != nullptr;
+
+  auto xfwd = [&](int nsub, int kx, int ky, int kz, void* out) {
+    P2Args a = base_args(e);
+    a.L = Xi;
+    a.nsub = nsub;
+    a.W = nlines;
+    a.in_os = Xi; a.in_is = 1;
+    a.out_os = (long long)nsub * Xi; a.out_is = 1;
+    a.rphase = nsub == 1 ? 1 : 0;
+    a.kx = kx; a.ky = ky; a.kz = kz;
+    const double bytes = (atoms ? site_bytes : si * N) + sa * N * nsub;
+    if (atoms) {
+      a.site_ptr = e->d_site_ptr;
+      a.site_cell = e->d_site_cell;
+      a.site_in = sites_in;
+      launch2<double, TC, TA, PM_FWD, true>(e, e->cxf_s, a, 1, 0, nullptr, out, K_XFWD, bytes);
+    } else {
+      launch2<TIN, TC, TA, PM_FWD, true>(e, e->cxf, a, 1, 0, e->F, out, K_XFWD, bytes);
+    }
+  };
+  auto xinv = [&](int nsub, int kx, int ky, int kz, int acc, const void* in) {
+    P2Args a = base_args(e);
+    a.L = Xi;
+    a.nsub = nsub;
+    a.W = nlines;
+    a.in_os = (long long)nsub * Xi; a.in_is = 1;
+    a.out_os = Xi; a.out_is = 1;
+    a.rphase = nsub == 1 ? 1 : 0;
+    a.kx = kx; a.ky = ky; a.kz = kz;
+    a.acc = acc;
+    if (atoms) {
+      a.site_ptr = e->d_site_ptr;
+      a.site_cell = e->d_site_cell;
+      a.site_out = sites_out;
+      const double bytes = sd * N * nsub + site_bytes + (acc ? 16.0 * e->n_sites : 0.0);
+      launch2<TD, TC, double, PM_INV, true>(e, e->cxi_s, a, 1, 0, in, nullptr, K_XINV, bytes);
+    } else {
+      const double bytes = sd * N * nsub + sv * N * (acc ? 2 : 1);
+      launch2<TD, TC, TV, PM_INV, true>(e, e->cxi, a, 1, 0, in, e->V, K_XINV, bytes);
+    }
+  };
+
+  if (e->shared()) {
+    const long long W = 2LL * Xi;
+    xfwd(2, 0, 0, 0, e->A);
+    {
+      P2Args a = base_args(e);
+      a.L = Yi; a.nsub = 2; a.W = (int)W;
+      a.in_os = (long long)Yi * W; a.in_is = W; a.out_os = 2LL * Yi * W; a.out_is = W;
+      launch2<TA, TC, TB, PM_FWD, false>(e, e->cyf, a, Zi, 1, e->A, e->B, K_YFWD, (sa * 2 + sb * 4) * N);
+    }
+    {
+      P2Args a = base_args(e);
+      a.L = Zi; a.nsub = 2; a.W = (int)W;
+      a.in_os = a.out_os = W; a.in_is = a.out_is = 2LL * Yi * W;
+      a.gxm = 1; a.gym = 1;
+      launch2<TB, TC, TB, PM_CONV, false>(e, e->czc, a, 2 * Yi, 2, e->B, e->B, K_ZCONV,
+                                          sb * 8 * N + (double)sizeof(TC) * (double)e->NG());
+    }
+    {
+      P2Args a = base_args(e);
+      a.L = Yi; a.nsub = 2; a.W = (int)W;
+      a.in_os = 2LL * Yi * W; a.in_is = W; a.out_os = (long long)Yi * W; a.out_is = W;
+      launch2<TB, TC, TD, PM_INV, false>(e, e->cyi, a, Zi, 1, e->B, e->D, K_YINV, (sb * 4 + sd * 2) * N);
+    }
+    xinv(2, 0, 0, 0, 0, e->D);
+  } else {
+    const long long W = Xi;
+    for (int round = 0; round < 8; ++round) {
+      const int kx = round & 1, ky = (round >> 1) & 1, kz = (round >> 2) & 1;
+      xfwd(1, kx, ky, kz, e->A);
+      P2Args a = base_args(e);
+      a.L = Yi; a.nsub = 1; a.W = (int)W;
+      a.in_os = a.out_os = (long long)Yi * W; a.in_is = a.out_is = W;
+      launch2<TA, TC, TA, PM_FWD, false>(e, e->cyf, a, Zi, 1, e->A, e->A, K_YFWD, 2 * sa * N);
+      P2Args c = base_args(e);
+      c.L = Zi; c.nsub = 1; c.W = (int)W;
+      c.in_os = c.out_os = W; c.in_is = c.out_is = (long long)Yi * W;
+      c.zpar = kz; c.gxm = 2; c.gxa = kx; c.gym = 2; c.gya = ky;
+      launch2<TA, TC, TA, PM_CONV, false>(e, e->czc, c, Yi, 2, e->A, e->A, K_ZCONV,
+                                          2 * sa * N + (double)sizeof(TC) * (double)e->NG());
+      launch2<TA, TC, TD, PM_INV, false>(e, e->cyi, a, Zi, 1, e->A, e->D, K_YINV, (sa + sd) * N);
+      xinv(1, kx, ky, kz, round > 0 ? 1 : 0, e->D);
+    }
+  }
+}
+
+// ===========================================================================
+// generic (Stockham, single precision type) path
+// ===========================================================================
 int tile_lines(const clat_engine* e, int nsub, int L, int cap = 64) {
   const int c = floor_pow2(std::max(1, e->tile_budget / (nsub * L)));
   return std::min(c, cap);
 }
 
-// ---------------------------------------------------------------------------
-// prepare(): input F -> potential V
-// ---------------------------------------------------------------------------
 template <typename T, int EMAX>
-void run_prepare(clat_engine* e) {
+void prepare_generic(clat_engine* e) {
   using CT = C2<T>;
   const int Xi = e->Xi, Yi = e->Yi, Zi = e->Zi;
   const size_t N = e->N();
   const double s = (double)sizeof(CT);
-  const CT* twx = (const CT*)e->tw[0];
-  const CT* twy = (const CT*)e->tw[1];
-  const CT* twz = (const CT*)e->tw[2];
-  const CT* hx = (const CT*)e->half[0];
-  const CT* hy = (const CT*)e->half[1];
-  const CT* hz = (const CT*)e->half[2];
-  const T* G = (sizeof(T) == 4) ? (const T*)(const void*)e->G32 : (const T*)(const void*)e->G64;
+  const CT* twx = tw_of<T>(e, 0);
+  const CT* twy = tw_of<T>(e, 1);
+  const CT* twz = tw_of<T>(e, 2);
+  const CT* hx = h_of<T>(e, 0);
+  const CT* hy = h_of<T>(e, 1);
+  const CT* hz = h_of<T>(e, 2);
+  const T* G = g_of<T>(e);
   CT* F = (CT*)e->F;
   CT* V = (CT*)e->V;
   CT* A = (CT*)e->A;
@@ -292,11 +544,11 @@ void run_prepare(clat_engine* e) {
     LAUNCH_CHECK();
     e->kend(K_XINV, s * (double)N * (1 + nsub + (acc ? 1 : 0)));
   };
-  auto col = [&](int mode, const AxisPlan& p, const CT* tw, const CT* h, ColArgs a, int n_outer,
-                 const CT* in, CT* out) {
+  auto col = [&](int mode, const AxisPlan& p, const CT* tw, const CT* h, ColArgs a, int n_outer, const CT* in,
+                 CT* out) {
     const int L = p.L;
     a.C = tile_lines(e, a.nsub, L, 64);
-    a.C = std::min(a.C, std::max(1, floor_pow2(a.W) * (a.W > floor_pow2(a.W) ? 2 : 1)));
+    a.C = std::min(a.C, ceil_pow2(a.W));
     a.lgC = log2i(a.C);
     const size_t lds = (size_t)a.nsub * a.C * L * sizeof(CT);
     dim3 grid((a.W + a.C - 1) / a.C, n_outer);
@@ -327,7 +579,7 @@ void run_prepare(clat_engine* e) {
   base.GX1 = Xi + 1;
   base.GY1 = Yi + 1;
 
-  if (e->sched == CLAT_SCHED_SHARED) {
+  if (e->shared()) {
     const long long W = 2LL * Xi;
     xfwd(2, 0, 0, 0, A);
     ColArgs a = base;
@@ -369,29 +621,81 @@ void run_prepare(clat_engine* e) {
   }
 }
 
-void prepare(clat_engine* e) {
-  if (e->prec == CLAT_PREC_FP32) run_prepare<float, 32>(e);
-  else run_prepare<double, 16>(e);
-}
-
-// zero the engine grid F
-void zero_F(clat_engine* e) {
-  if (e->prec == CLAT_PREC_FP32)
-    hipLaunchKernelGGL(k_zero<float>, dim3(grid1d(e->N())), dim3(256), 0, e->stream, (float2*)e->F, e->N());
+// ===========================================================================
+// buffers and prepare dispatch
+// ===========================================================================
+void zero_grid(clat_engine* e, void* p, size_t n, size_t csz) {
+  if (csz == 8)
+    hipLaunchKernelGGL(k_zero<float>, dim3(grid1d(n)), dim3(256), 0, e->stream, (float2*)p, n);
   else
-    hipLaunchKernelGGL(k_zero<double>, dim3(grid1d(e->N())), dim3(256), 0, e->stream, (double2*)e->F, e->N());
+    hipLaunchKernelGGL(k_zero<double>, dim3(grid1d(n)), dim3(256), 0, e->stream, (double2*)p, n);
   LAUNCH_CHECK();
 }
 
-// ---------------------------------------------------------------------------
+// dense F and V of the grid protocol (and of the generic path)
+void ensure_dense(clat_engine* e) {
+  if (!e->F) {
+    e->F = e->dalloc(e->f_bytes);
+    zero_grid(e, e->F, e->N(), e->s_in);
+  }
+  if (!e->V) e->V = e->dalloc(e->v_bytes);
+}
+
+// dense F -> dense V
+void prepare_dense(clat_engine* e) {
+  ensure_dense(e);
+  if (e->fast) {
+    if (e->rprec == CLAT_PREC_FP64) prepare_fast<double, double, double, double, double, double>(e, nullptr, nullptr);
+    else if (e->rprec == CLAT_PREC_FP32) prepare_fast<float, float, float, float, float, float>(e, nullptr, nullptr);
+    else prepare_fast<double, float, float, double, double, double>(e, nullptr, nullptr);
+  } else {
+    if (e->rprec == CLAT_PREC_FP32) prepare_generic<float, 32>(e);
+    else prepare_generic<double, 16>(e);
+  }
+}
+
+// pair density at the sites -> potential at the sites (FP64)
+void prepare_sites(clat_engine* e, const double2* rho_sites, double2* v_sites) {
+  if (e->fast) {
+    if (e->rprec == CLAT_PREC_FP64) prepare_fast<double, double, double, double, double, double>(e, rho_sites, v_sites);
+    else if (e->rprec == CLAT_PREC_FP32) prepare_fast<float, float, float, float, float, float>(e, rho_sites, v_sites);
+    else prepare_fast<double, float, float, double, double, double>(e, rho_sites, v_sites);
+    return;
+  }
+  ensure_dense(e);
+  zero_grid(e, e->F, e->N(), e->s_in);
+  e->kbegin();
+  if (e->s_in == 8)
+    hipLaunchKernelGGL(k_scatter<float>, dim3(grid1d(e->n_sites)), dim3(256), 0, e->stream, (float2*)e->F,
+                       e->d_site_cell, rho_sites, e->n_sites);
+  else
+    hipLaunchKernelGGL(k_scatter<double>, dim3(grid1d(e->n_sites)), dim3(256), 0, e->stream, (double2*)e->F,
+                       e->d_site_cell, rho_sites, e->n_sites);
+  LAUNCH_CHECK();
+  e->kend(K_SCATTER, (double)e->n_sites * (20.0 + (double)e->s_in));
+  if (e->rprec == CLAT_PREC_FP32) prepare_generic<float, 32>(e);
+  else prepare_generic<double, 16>(e);
+  e->kbegin();
+  if (e->s_v == 8)
+    hipLaunchKernelGGL(k_gather<float>, dim3(grid1d(e->n_sites)), dim3(256), 0, e->stream, (const float2*)e->V,
+                       e->d_site_cell, v_sites, e->n_sites);
+  else
+    hipLaunchKernelGGL(k_gather<double>, dim3(grid1d(e->n_sites)), dim3(256), 0, e->stream, (const double2*)e->V,
+                       e->d_site_cell, v_sites, e->n_sites);
+  LAUNCH_CHECK();
+  e->kend(K_GATHER, (double)e->n_sites * (20.0 + (double)e->s_v));
+}
+
+// ===========================================================================
 // interaction table: G -> DCT-I -> /(8 Xi Yi Zi)
-// ---------------------------------------------------------------------------
+// ===========================================================================
 void dct_axis(clat_engine* e, int axis, double scale) {
   const int X1 = e->Xi + 1, Y1 = e->Yi + 1, Z1 = e->Zi + 1;
   const AxisPlan& p = axis == 0 ? e->dx : (axis == 1 ? e->dy : e->dz);
   const int L2 = p.L;
   const int C = std::min(64, floor_pow2(std::max(1, 4096 / L2)));
   const size_t lds = (size_t)C * L2 * sizeof(double2);
+  e->kbegin();
   if (axis == 0) {
     const int ncols = Y1 * Z1;
     dim3 grid((ncols + C - 1) / C);
@@ -409,13 +713,14 @@ void dct_axis(clat_engine* e, int axis, double scale) {
                        log2i(C), 0LL, (long long)X1 * Y1, 1LL, ncols, scale);
   }
   LAUNCH_CHECK();
+  e->kend(K_DCT, 16.0 * (double)e->NG());
 }
 
 void finish_interaction(clat_engine* e) {
   dct_axis(e, 0, 1.0);
   dct_axis(e, 1, 1.0);
   dct_axis(e, 2, 1.0 / (8.0 * (double)e->Xi * (double)e->Yi * (double)e->Zi));
-  if (e->prec == CLAT_PREC_FP32) {
+  if (e->G32) {
     hipLaunchKernelGGL(k_d2f, dim3(grid1d(e->NG())), dim3(256), 0, e->stream, e->G64, e->G32, e->NG());
     LAUNCH_CHECK();
   }
@@ -441,12 +746,12 @@ void build_table(clat_engine* e, GArgs g) {
   float ms = 0.f;
   HIPX(hipEventElapsedTime(&ms, e->stage_ev[0], e->stage_ev[5]));
   e->stage_ms[0] = ms;
+  e->kcollect();
 }
 
 void ensure_dpart(clat_engine* e, size_t n) {
   if (e->dpart_cap < n) {
-    void* p = e->dpart;
-    e->dfree(p, e->dpart_cap * sizeof(double));
+    e->dfree(e->dpart, e->dpart_cap * sizeof(double));
     e->dpart = (double*)e->dalloc(n * sizeof(double));
     e->dpart_cap = n;
   }
@@ -464,18 +769,17 @@ double nsum(const double* v, size_t n, size_t stride) {
   return s + c;
 }
 
-// ---------------------------------------------------------------------------
+// ===========================================================================
 // atom-site path helpers
-// ---------------------------------------------------------------------------
+// ===========================================================================
 void build_products(clat_engine* e) {
   const int P = e->n_states * (e->n_states + 1) / 2;
   if (e->rho_states == e->n_states && e->rho) return;
-  void* r = e->rho;
-  e->dfree(r, (size_t)(e->rho_states > 0 ? e->rho_states * (e->rho_states + 1) / 2 : 0) * e->n_sites *
-                  sizeof(double2));
-  e->rho = nullptr;
-  e->rho = (double2*)e->dalloc((size_t)P * e->n_sites * sizeof(double2));
+  e->dfree(e->rho, e->rho_bytes);
+  e->rho_bytes = (size_t)P * e->n_sites * sizeof(double2);
+  e->rho = (double2*)e->dalloc(e->rho_bytes);
   e->rho_states = e->n_states;
+  if (e->n_sites == 0) return;
   const size_t per_site = (size_t)e->n_states * e->n_orb * sizeof(double2);
   int A = (int)std::min<size_t>(64, 49152 / per_site);
   e->kbegin();
@@ -491,31 +795,25 @@ void build_products(clat_engine* e) {
   e->kend(K_PRODUCTS, (double)e->n_sites * (double)per_site + (double)P * e->n_sites * 16.0);
 }
 
-// F <- rho of product p at every site (grid zeroed first)
-void scatter_product(clat_engine* e, int p) {
-  zero_F(e);
-  const double2* r = e->rho + (size_t)p * e->n_sites;
-  e->kbegin();
-  if (e->prec == CLAT_PREC_FP32)
-    hipLaunchKernelGGL(k_scatter<float>, dim3(grid1d(e->n_sites)), dim3(256), 0, e->stream, (float2*)e->F,
-                       e->d_site_cell, r, e->n_sites);
-  else
-    hipLaunchKernelGGL(k_scatter<double>, dim3(grid1d(e->n_sites)), dim3(256), 0, e->stream, (double2*)e->F,
-                       e->d_site_cell, r, e->n_sites);
-  LAUNCH_CHECK();
-  e->kend(K_SCATTER, (double)e->n_sites * (16.0 + 4.0 + (double)e->csz));
+// out[a] = vs[atom_site[a]]
+__global__ void k_atoms_from_sites(const double2* __restrict__ vs, const int32_t* __restrict__ atom_site,
+                                   double2* __restrict__ out, int64_t n) {
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x)
+    out[t] = vs[atom_site[t]];
 }
 
-void gather_V(clat_engine* e, const int32_t* cells, int64_t n, double2* out) {
-  e->kbegin();
-  if (e->prec == CLAT_PREC_FP32)
-    hipLaunchKernelGGL(k_gather<float>, dim3(grid1d(n)), dim3(256), 0, e->stream, (const float2*)e->V, cells,
-                       out, n);
-  else
-    hipLaunchKernelGGL(k_gather<double>, dim3(grid1d(n)), dim3(256), 0, e->stream, (const double2*)e->V, cells,
-                       out, n);
-  LAUNCH_CHECK();
-  e->kend(K_GATHER, (double)n * (16.0 + 4.0 + (double)e->csz));
+void free_atoms(clat_engine* e) {
+  e->dfree(e->d_site_atom, (size_t)e->n_sites * 4);
+  e->dfree(e->d_site_cell, (size_t)e->n_sites * 4);
+  e->dfree(e->d_site_ptr, ((size_t)e->Yi * e->Zi + 1) * 4);
+  e->dfree(e->d_atom_site, (size_t)e->n_atoms * 4);
+  e->dfree(e->coeffs, (size_t)e->cap_states * e->n_atoms * e->n_orb * 16);
+  e->dfree(e->rho, e->rho_bytes);
+  e->rho_bytes = 0;
+  e->rho_states = -1;
+  e->n_states = e->cap_states = 0;
+  e->n_sites = 0;
+  e->n_atoms = 0;
 }
 
 }  // namespace
@@ -527,7 +825,7 @@ extern "C" {
 
 const char* clat_last_error(void) { return g_err.c_str(); }
 
-const char* clat_version(void) { return "clat 1.0 (gfx950)"; }
+const char* clat_version(void) { return "clat 1.1 (gfx950)"; }
 
 int clat_round_up(int n, int c, const int* primes, int n_primes, int* out) {
   return guarded([&] {
@@ -608,7 +906,7 @@ int clat_create(int X, int Y, int Z, int precision, int schedule, clat_engine** 
   int rc = guarded([&] {
     if (X <= 0 || Y <= 0 || Z <= 0) throw host::Error(CLAT_EINVAL, "dimensions must be positive");
     if ((double)X * Y * Z >= 2147483647.0) throw host::Error(CLAT_EINVAL, "dimensions are too large");
-    if (precision != CLAT_PREC_FP64 && precision != CLAT_PREC_FP32)
+    if (precision != CLAT_PREC_FP64 && precision != CLAT_PREC_FP32 && precision != CLAT_PREC_FP32X)
       throw host::Error(CLAT_EINVAL, "unknown precision");
     if (schedule != CLAT_SCHED_SHARED && schedule != CLAT_SCHED_ROUNDS)
       throw host::Error(CLAT_EINVAL, "unknown schedule");
@@ -617,55 +915,90 @@ int clat_create(int X, int Y, int Z, int precision, int schedule, clat_engine** 
       throw host::Error(CLAT_EDEVICE, "no HIP device available");
     HIPX(hipGetDevice(&e->device));
     e->X = X; e->Y = Y; e->Z = Z;
-    e->Xi = host::round2357(X); e->Yi = host::round2357(Y); e->Zi = host::round2357(Z);
-    if ((double)e->Xi * e->Yi * e->Zi * (schedule == CLAT_SCHED_SHARED ? 4.0 : 1.0) >= 4294967295.0)
+    e->Xi = engine_len(X); e->Yi = engine_len(Y); e->Zi = engine_len(Z);
+    if ((double)e->Xi * e->Yi * e->Zi * 4.0 >= 4294967295.0)
       throw host::Error(CLAT_EUNSUP, "grid too large for one engine");
     e->prec = precision;
     e->sched = schedule;
-    e->csz = precision == CLAT_PREC_FP32 ? 8 : 16;
-    e->tile_budget = precision == CLAT_PREC_FP32 ? 32 * NT : 16 * NT;
-    if (const char* tb = std::getenv("CLAT_TILE_ELEMS")) {
-      const int v = std::atoi(tb);
-      if (v >= 64 && v <= e->tile_budget) e->tile_budget = v;
+    int r1, r2;
+    e->fast = split2(e->Xi, r1, r2) && split2(e->Yi, r1, r2) && split2(e->Zi, r1, r2) &&
+              e->Xi <= 400 && e->Yi <= 400 && e->Zi <= 400;
+    if (const char* g = std::getenv("CLAT_GENERIC"))
+      if (std::atoi(g)) e->fast = false;
+    // the mixed mode needs the two-level kernels; otherwise it runs in FP64
+    e->rprec = (precision == CLAT_PREC_FP32X && !e->fast) ? CLAT_PREC_FP64 : precision;
+    if (const char* lb = std::getenv("CLAT_LDS_BYTES")) {
+      const long v = std::atol(lb);
+      if (v >= 4096 && v <= 163840) e->lds_budget = (size_t)v;
     }
+    const int nsub = schedule == CLAT_SCHED_SHARED ? 2 : 1;
+    const size_t dbl = 16, flt = 8;
+    switch (e->rprec) {
+      case CLAT_PREC_FP64: e->s_in = e->s_a = e->s_b = e->s_d = e->s_v = dbl; break;
+      case CLAT_PREC_FP32: e->s_in = e->s_a = e->s_b = e->s_d = e->s_v = flt; break;
+      default: e->s_in = dbl; e->s_a = flt; e->s_b = flt; e->s_d = dbl; e->s_v = dbl; break;
+    }
+    const size_t tc = e->rprec == CLAT_PREC_FP32 ? flt : dbl;  // arithmetic type
     const int dims[3] = {e->Xi, e->Yi, e->Zi};
     AxisPlan* lp[3] = {&e->px, &e->py, &e->pz};
     AxisPlan* dp[3] = {&e->dx, &e->dy, &e->dz};
+    e->tile_budget = e->rprec == CLAT_PREC_FP32 ? 32 * NT : 16 * NT;
     for (int d = 0; d < 3; ++d) {
       if (!make_plan(dims[d], *lp[d]) || !make_plan(2 * dims[d], *dp[d]))
         throw host::Error(CLAT_EUNSUP, "axis length has no radix plan");
-      const int nsub = schedule == CLAT_SCHED_SHARED ? 2 : 1;
-      if (nsub * dims[d] > e->tile_budget || 2 * dims[d] > 16 * NT)
+      if (!e->fast && nsub * dims[d] > e->tile_budget)
         throw host::Error(CLAT_EUNSUP, "axis length exceeds the single-tile FFT limit");
+      if (2 * dims[d] > 16 * NT) throw host::Error(CLAT_EUNSUP, "axis length exceeds the DCT-I tile limit");
+    }
+    if (e->fast) {
+      const int W = nsub * e->Xi;
+      e->cxf = cfg_line(e->Xi, nsub, tc, false, e->lds_budget);
+      e->cxf_s = cfg_line(e->Xi, nsub, tc, true, e->lds_budget);
+      e->cxi = cfg_line(e->Xi, nsub, tc, false, e->lds_budget);
+      e->cxi_s = cfg_line(e->Xi, nsub, tc, true, e->lds_budget);
+      e->cyf = cfg_col(e->Yi, nsub, tc, W, e->lds_budget);
+      e->czc = cfg_col(e->Zi, nsub, tc, W, e->lds_budget);
+      e->cyi = cfg_col(e->Yi, nsub, tc, W, e->lds_budget);
+      const PassCfg* all[] = {&e->cxf, &e->cxf_s, &e->cxi, &e->cxi_s, &e->cyf, &e->czc, &e->cyi};
+      for (const PassCfg* p : all)
+        if (p->lds > 65536) throw host::Error(CLAT_EUNSUP, "pass tile exceeds the LDS budget");
+      if (e->cxf_s.C * e->cxf_s.R1 > e->cxf_s.NT || e->cxi_s.C * e->cxi_s.R1 > e->cxi_s.NT)
+        throw host::Error(CLAT_EUNSUP, "site pass shape");
     }
     HIPX(hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking));
     for (int i = 0; i < 6; ++i) HIPX(hipEventCreate(&e->stage_ev[i]));
-    // twiddle and half-shift tables
     for (int d = 0; d < 3; ++d) {
       const int L = dims[d];
       std::vector<double> tw = unit_roots(L, L), hf = unit_roots(L, 2 * L), dt = unit_roots(2 * L, 2 * L);
-      if (precision == CLAT_PREC_FP32) {
-        std::vector<float> a(tw.begin(), tw.end()), b(hf.begin(), hf.end());
-        e->tw[d] = e->dalloc(a.size() * sizeof(float));
-        e->half[d] = e->dalloc(b.size() * sizeof(float));
-        HIPX(hipMemcpy(e->tw[d], a.data(), a.size() * sizeof(float), hipMemcpyHostToDevice));
-        HIPX(hipMemcpy(e->half[d], b.data(), b.size() * sizeof(float), hipMemcpyHostToDevice));
-      } else {
-        e->tw[d] = e->dalloc(tw.size() * sizeof(double));
-        e->half[d] = e->dalloc(hf.size() * sizeof(double));
-        HIPX(hipMemcpy(e->tw[d], tw.data(), tw.size() * sizeof(double), hipMemcpyHostToDevice));
-        HIPX(hipMemcpy(e->half[d], hf.data(), hf.size() * sizeof(double), hipMemcpyHostToDevice));
-      }
+      std::vector<float> a(tw.begin(), tw.end()), b(hf.begin(), hf.end());
+      e->tw32[d] = e->dalloc(a.size() * sizeof(float));
+      e->h32[d] = e->dalloc(b.size() * sizeof(float));
+      e->tw64[d] = e->dalloc(tw.size() * sizeof(double));
+      e->h64[d] = e->dalloc(hf.size() * sizeof(double));
       e->dtw[d] = (double2*)e->dalloc(dt.size() * sizeof(double));
+      HIPX(hipMemcpy(e->tw32[d], a.data(), a.size() * sizeof(float), hipMemcpyHostToDevice));
+      HIPX(hipMemcpy(e->h32[d], b.data(), b.size() * sizeof(float), hipMemcpyHostToDevice));
+      HIPX(hipMemcpy(e->tw64[d], tw.data(), tw.size() * sizeof(double), hipMemcpyHostToDevice));
+      HIPX(hipMemcpy(e->h64[d], hf.data(), hf.size() * sizeof(double), hipMemcpyHostToDevice));
       HIPX(hipMemcpy(e->dtw[d], dt.data(), dt.size() * sizeof(double), hipMemcpyHostToDevice));
     }
     e->G64 = (double*)e->dalloc(e->NG() * sizeof(double));
-    if (precision == CLAT_PREC_FP32) e->G32 = (float*)e->dalloc(e->NG() * sizeof(float));
-    e->F = e->dalloc(e->N() * e->csz);
-    e->V = e->dalloc(e->N() * e->csz);
-    e->A = e->dalloc(e->a_elems() * e->csz);
-    if (e->b_elems()) e->B = e->dalloc(e->b_elems() * e->csz);
-    zero_F(e);
+    if (e->rprec == CLAT_PREC_FP32) e->G32 = (float*)e->dalloc(e->NG() * sizeof(float));
+    const size_t N = e->N();
+    e->f_bytes = N * e->s_in;
+    e->v_bytes = N * e->s_v;
+    e->a_bytes = (e->shared() ? 2 : 1) * N * e->s_a;
+    e->A = e->dalloc(e->a_bytes);
+    if (e->shared()) {
+      e->b_bytes = 4 * N * e->s_b;
+      e->B = e->dalloc(e->b_bytes);
+    }
+    if (e->s_d == e->s_a) {
+      e->D = e->A;  // the y-inverse output reuses the x-forward buffer
+    } else {
+      e->d_bytes = (e->shared() ? 2 : 1) * N * e->s_d;
+      e->D = e->dalloc(e->d_bytes);
+    }
     HIPX(hipStreamSynchronize(e->stream));
   });
   if (rc != CLAT_OK) {
@@ -679,9 +1012,11 @@ int clat_create(int X, int Y, int Z, int precision, int schedule, clat_engine** 
 int clat_destroy(clat_engine* e) {
   if (!e) return CLAT_OK;
   if (e->stream) (void)hipStreamSynchronize(e->stream);
-  void* ptrs[] = {e->tw[0], e->tw[1], e->tw[2], e->half[0], e->half[1], e->half[2], e->dtw[0], e->dtw[1],
-                  e->dtw[2], e->G64, e->G32, e->F, e->V, e->A, e->B, e->Fin, e->dpart, e->d_cell,
-                  e->d_site_atom, e->d_site_cell, e->coeffs, e->rho, e->Vat, e->part, e->ovl};
+  void* ptrs[] = {e->tw32[0], e->tw32[1], e->tw32[2], e->h32[0], e->h32[1], e->h32[2], e->tw64[0], e->tw64[1],
+                  e->tw64[2], e->h64[0], e->h64[1], e->h64[2], e->dtw[0], e->dtw[1], e->dtw[2], e->G64,
+                  e->G32, e->F, e->V, e->A, e->B, (e->D != e->A) ? e->D : nullptr, e->Fin, e->dpart,
+                  e->d_site_atom, e->d_site_cell, e->d_site_ptr, e->d_atom_site, e->coeffs, e->rho, e->Vat,
+                  e->part, e->ovl};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   for (auto ev : e->stage_ev)
@@ -736,30 +1071,30 @@ int clat_set_interaction_table(clat_engine* e, const double* G) {
   return guarded([&] {
     const size_t nu = (size_t)(e->X + 1) * (e->Y + 1) * (e->Z + 1);
     double* tmp = (double*)e->dalloc(nu * sizeof(double));
-    try {
-      HIPX(hipMemcpyAsync(tmp, G, nu * sizeof(double), hipMemcpyHostToDevice, e->stream));
+    hipError_t ce = hipMemcpyAsync(tmp, G, nu * sizeof(double), hipMemcpyHostToDevice, e->stream);
+    if (ce == hipSuccess) {
       GArgs g = base_gargs(e);
       hipLaunchKernelGGL(k_gpad, dim3(grid1d(e->NG())), dim3(256), 0, e->stream, tmp, e->G64, g);
-      LAUNCH_CHECK();
-      finish_interaction(e);
-      HIPX(hipStreamSynchronize(e->stream));
-    } catch (...) {
-      void* t = tmp;
-      e->dfree(t, nu * sizeof(double));
-      throw;
+      ce = hipGetLastError();
     }
-    void* t = tmp;
-    e->dfree(t, nu * sizeof(double));
+    hipError_t se = hipStreamSynchronize(e->stream);
+    e->dfree(tmp, nu * sizeof(double));
+    HIPX(ce);
+    HIPX(se);
+    finish_interaction(e);
+    HIPX(hipStreamSynchronize(e->stream));
+    e->kcollect();
   });
 }
 
 int clat_set_input(clat_engine* e, const double* F) {
   if (!e || !F) return set_err(CLAT_EINVAL, "invalid arguments");
   return guarded([&] {
+    ensure_dense(e);
     if (!e->Fin) e->Fin = (double2*)e->dalloc(e->NU() * sizeof(double2));
     HIPX(hipMemcpyAsync(e->Fin, F, e->NU() * sizeof(double2), hipMemcpyHostToDevice, e->stream));
-    zero_F(e);
-    if (e->prec == CLAT_PREC_FP32)
+    zero_grid(e, e->F, e->N(), e->s_in);
+    if (e->s_in == 8)
       hipLaunchKernelGGL(k_grid_in<float>, dim3(grid1d(e->NU())), dim3(256), 0, e->stream, e->Fin,
                          (float2*)e->F, e->X, e->Y, e->Z, e->Xi, e->Yi);
     else
@@ -775,7 +1110,7 @@ int clat_prepare(clat_engine* e) {
   if (!e) return set_err(CLAT_EINVAL, "invalid arguments");
   if (!e->initialized) return set_err(CLAT_ESTATE, "calculator is not initialized");
   return guarded([&] {
-    prepare(e);
+    prepare_dense(e);
     e->kcollect();
     HIPX(hipStreamSynchronize(e->stream));
     e->prepared = true;
@@ -789,7 +1124,7 @@ int clat_calculate(clat_engine* e, double result[2]) {
   return guarded([&] {
     const unsigned nb = std::min<unsigned>(grid1d(e->NU()), 2048);
     ensure_dpart(e, 2 * (size_t)nb);
-    if (e->prec == CLAT_PREC_FP32)
+    if (e->s_v == 8)
       hipLaunchKernelGGL(k_grid_dot<float>, dim3(nb), dim3(NT), 0, e->stream, (const float2*)e->V, e->Fin, e->X,
                          e->Y, e->Z, e->Xi, e->Yi, e->dpart);
     else
@@ -809,7 +1144,7 @@ int clat_get_potential(clat_engine* e, double* V) {
   if (!e->prepared) return set_err(CLAT_ESTATE, "prepare() has not been called");
   return guarded([&] {
     double2* tmp = (double2*)e->dalloc(e->NU() * sizeof(double2));
-    if (e->prec == CLAT_PREC_FP32)
+    if (e->s_v == 8)
       hipLaunchKernelGGL(k_grid_out<float>, dim3(grid1d(e->NU())), dim3(256), 0, e->stream, (const float2*)e->V,
                          tmp, e->X, e->Y, e->Z, e->Xi, e->Yi);
     else
@@ -818,8 +1153,7 @@ int clat_get_potential(clat_engine* e, double* V) {
     hipError_t le = hipGetLastError();
     hipError_t ce = hipMemcpyAsync(V, tmp, e->NU() * sizeof(double2), hipMemcpyDeviceToHost, e->stream);
     hipError_t se = hipStreamSynchronize(e->stream);
-    void* t = tmp;
-    e->dfree(t, e->NU() * sizeof(double2));
+    e->dfree(tmp, e->NU() * sizeof(double2));
     HIPX(le);
     HIPX(ce);
     HIPX(se);
@@ -829,6 +1163,7 @@ int clat_get_potential(clat_engine* e, double* V) {
 int clat_set_atoms(clat_engine* e, const int32_t* cell, int64_t n_atoms, int n_orbitals) {
   if (!e || (!cell && n_atoms > 0) || n_atoms < 0) return set_err(CLAT_EINVAL, "invalid arguments");
   if (n_orbitals <= 0) return set_err(CLAT_EINVAL, "invalid value for orbitals");
+  if (n_atoms >= 2147483647) return set_err(CLAT_EUNSUP, "too many atoms");
   return guarded([&] {
     const int64_t NU = (int64_t)e->NU();
     // effective sites: the last atom of every occupied cell, ordered by cell
@@ -837,38 +1172,31 @@ int clat_set_atoms(clat_engine* e, const int32_t* cell, int64_t n_atoms, int n_o
       if (cell[a] < 0 || cell[a] >= NU) throw host::Error(CLAT_EINVAL, "atom cell index outside the grid");
       last[(size_t)cell[a]] = (int32_t)a;
     }
-    std::vector<int32_t> site_atom, site_cell, all_cell((size_t)n_atoms);
-    for (int64_t c = 0; c < NU; ++c)
-      if (last[(size_t)c] >= 0) {
-        site_atom.push_back(last[(size_t)c]);
-        const int ix = (int)(c % e->X), iy = (int)((c / e->X) % e->Y), iz = (int)(c / ((int64_t)e->X * e->Y));
-        site_cell.push_back((int32_t)(((int64_t)iz * e->Yi + iy) * e->Xi + ix));
-      }
-    for (int64_t a = 0; a < n_atoms; ++a) {
-      const int64_t c = cell[a];
+    const int64_t nlines = (int64_t)e->Yi * e->Zi;
+    std::vector<int32_t> site_atom, site_cell, site_ptr((size_t)nlines + 1, 0), atom_site((size_t)n_atoms);
+    for (int64_t c = 0; c < NU; ++c) {
+      if (last[(size_t)c] < 0) continue;
       const int ix = (int)(c % e->X), iy = (int)((c / e->X) % e->Y), iz = (int)(c / ((int64_t)e->X * e->Y));
-      all_cell[(size_t)a] = (int32_t)(((int64_t)iz * e->Yi + iy) * e->Xi + ix);
+      const int64_t line = (int64_t)iz * e->Yi + iy;
+      site_ptr[(size_t)line + 1] += 1;
+      site_atom.push_back(last[(size_t)c]);
+      site_cell.push_back((int32_t)(line * e->Xi + ix));
+      last[(size_t)c] = (int32_t)(site_atom.size() - 1);  // cell -> site from here on
     }
-    // drop previous atoms, states and derived buffers
-    void* p;
-    p = e->d_cell; e->dfree(p, (size_t)e->n_atoms * 4); e->d_cell = nullptr;
-    p = e->d_site_atom; e->dfree(p, (size_t)e->n_sites * 4); e->d_site_atom = nullptr;
-    p = e->d_site_cell; e->dfree(p, (size_t)e->n_sites * 4); e->d_site_cell = nullptr;
-    p = e->coeffs; e->dfree(p, (size_t)e->cap_states * e->n_atoms * e->n_orb * 16); e->coeffs = nullptr;
-    p = e->rho;
-    e->dfree(p, (size_t)(e->rho_states > 0 ? e->rho_states * (e->rho_states + 1) / 2 : 0) * e->n_sites * 16);
-    e->rho = nullptr;
-    e->rho_states = -1;
-    e->n_states = e->cap_states = 0;
+    for (int64_t l = 0; l < nlines; ++l) site_ptr[(size_t)l + 1] += site_ptr[(size_t)l];
+    for (int64_t a = 0; a < n_atoms; ++a) atom_site[(size_t)a] = last[(size_t)cell[a]];
+    free_atoms(e);
     e->n_atoms = n_atoms;
     e->n_orb = n_orbitals;
     e->n_sites = (int64_t)site_atom.size();
-    e->d_cell = (int32_t*)e->dalloc((size_t)n_atoms * 4);
     e->d_site_atom = (int32_t*)e->dalloc((size_t)e->n_sites * 4);
     e->d_site_cell = (int32_t*)e->dalloc((size_t)e->n_sites * 4);
-    HIPX(hipMemcpy(e->d_cell, all_cell.data(), (size_t)n_atoms * 4, hipMemcpyHostToDevice));
+    e->d_site_ptr = (int32_t*)e->dalloc(((size_t)nlines + 1) * 4);
+    e->d_atom_site = (int32_t*)e->dalloc((size_t)n_atoms * 4);
     HIPX(hipMemcpy(e->d_site_atom, site_atom.data(), (size_t)e->n_sites * 4, hipMemcpyHostToDevice));
     HIPX(hipMemcpy(e->d_site_cell, site_cell.data(), (size_t)e->n_sites * 4, hipMemcpyHostToDevice));
+    HIPX(hipMemcpy(e->d_site_ptr, site_ptr.data(), ((size_t)nlines + 1) * 4, hipMemcpyHostToDevice));
+    HIPX(hipMemcpy(e->d_atom_site, atom_site.data(), (size_t)n_atoms * 4, hipMemcpyHostToDevice));
   });
 }
 
@@ -881,19 +1209,20 @@ int clat_add_state(clat_engine* e, const double* coeffs) {
       const int ncap = std::max(4, 2 * e->cap_states);
       double2* nc = (double2*)e->dalloc(per * ncap);
       if (e->n_states) HIPX(hipMemcpy(nc, e->coeffs, per * e->n_states, hipMemcpyDeviceToDevice));
-      void* p = e->coeffs;
-      e->dfree(p, per * e->cap_states);
+      e->dfree(e->coeffs, per * e->cap_states);
       e->coeffs = nc;
       e->cap_states = ncap;
     }
     HIPX(hipMemcpy((char*)e->coeffs + per * e->n_states, coeffs, per, hipMemcpyHostToDevice));
     e->n_states += 1;
+    e->rho_states = -1;
   });
 }
 
 int clat_clear_states(clat_engine* e) {
   if (!e) return set_err(CLAT_EINVAL, "invalid arguments");
   e->n_states = 0;
+  e->rho_states = -1;
   return CLAT_OK;
 }
 
@@ -915,18 +1244,14 @@ int clat_compute_integrals(clat_engine* e, const int32_t* quads, int64_t n, int 
     HIPX(hipEventRecord(e->stage_ev[1], e->stream));
     build_products(e);
     HIPX(hipEventRecord(e->stage_ev[2], e->stream));
-    const size_t vat_need = (size_t)std::max(1, nL) * e->n_sites;
+    const size_t vat_need = (size_t)std::max(1, nL) * (size_t)std::max<int64_t>(1, e->n_sites);
     if (e->vat_cap < vat_need) {
-      void* p = e->Vat;
-      e->dfree(p, e->vat_cap * sizeof(double2));
+      e->dfree(e->Vat, e->vat_cap * sizeof(double2));
       e->Vat = (double2*)e->dalloc(vat_need * sizeof(double2));
       e->vat_cap = vat_need;
     }
-    for (int sl = 0; sl < nL; ++sl) {
-      scatter_product(e, slots[(size_t)sl]);
-      prepare(e);
-      gather_V(e, e->d_site_cell, e->n_sites, e->Vat + (size_t)sl * e->n_sites);
-    }
+    for (int sl = 0; sl < nL && e->n_sites > 0; ++sl)
+      prepare_sites(e, e->rho + (size_t)slots[(size_t)sl] * e->n_sites, e->Vat + (size_t)sl * e->n_sites);
     HIPX(hipEventRecord(e->stage_ev[3], e->stream));
     // overlaps of every (slot, product) pair
     const int tl = (nL + OV_T - 1) / OV_T, tr = (P + OV_T - 1) / OV_T;
@@ -934,18 +1259,16 @@ int clat_compute_integrals(clat_engine* e, const int32_t* quads, int64_t n, int 
     n_chunks = std::min<int64_t>(n_chunks, (e->n_sites + OV_S - 1) / OV_S);
     n_chunks = std::max<int64_t>(1, n_chunks);
     int64_t chunk = (e->n_sites + n_chunks - 1) / n_chunks;
-    chunk = (chunk + OV_S - 1) / OV_S * OV_S;
+    chunk = std::max<int64_t>(OV_S, (chunk + OV_S - 1) / OV_S * OV_S);
     n_chunks = std::max<int64_t>(1, (e->n_sites + chunk - 1) / chunk);
     const size_t n_out = (size_t)std::max(1, nL) * P * 4;
     if (e->part_cap < n_out * n_chunks) {
-      void* p = e->part;
-      e->dfree(p, e->part_cap * sizeof(double));
+      e->dfree(e->part, e->part_cap * sizeof(double));
       e->part = (double*)e->dalloc(n_out * n_chunks * sizeof(double));
       e->part_cap = n_out * n_chunks;
     }
     if (e->ovl_cap < n_out) {
-      void* p = e->ovl;
-      e->dfree(p, e->ovl_cap * sizeof(double));
+      e->dfree(e->ovl, e->ovl_cap * sizeof(double));
       e->ovl = (double*)e->dalloc(n_out * sizeof(double));
       e->ovl_cap = n_out;
     }
@@ -996,14 +1319,27 @@ int clat_compute_potential_at_atoms(clat_engine* e, int state, double* v) {
   return guarded([&] {
     build_products(e);
     const int f = state - 1;
-    scatter_product(e, f * (f + 1) / 2 + f);
-    prepare(e);
-    double2* tmp = (double2*)e->dalloc((size_t)std::max<int64_t>(1, e->n_atoms) * sizeof(double2));
-    gather_V(e, e->d_cell, e->n_atoms, tmp);
-    hipError_t ce = hipMemcpyAsync(v, tmp, (size_t)e->n_atoms * sizeof(double2), hipMemcpyDeviceToHost, e->stream);
+    const size_t ns = (size_t)std::max<int64_t>(1, e->n_sites), na = (size_t)std::max<int64_t>(1, e->n_atoms);
+    double2* vs = (double2*)e->dalloc(ns * sizeof(double2));
+    double2* va = (double2*)e->dalloc(na * sizeof(double2));
+    hipError_t ce = hipSuccess;
+    try {
+      if (e->n_sites > 0) prepare_sites(e, e->rho + (size_t)(f * (f + 1) / 2 + f) * e->n_sites, vs);
+      if (e->n_atoms > 0) {
+        hipLaunchKernelGGL(k_atoms_from_sites, dim3(grid1d(e->n_atoms)), dim3(256), 0, e->stream, vs,
+                           e->d_atom_site, va, e->n_atoms);
+        LAUNCH_CHECK();
+        ce = hipMemcpyAsync(v, va, (size_t)e->n_atoms * sizeof(double2), hipMemcpyDeviceToHost, e->stream);
+      }
+    } catch (...) {
+      (void)hipStreamSynchronize(e->stream);
+      e->dfree(vs, ns * sizeof(double2));
+      e->dfree(va, na * sizeof(double2));
+      throw;
+    }
     hipError_t se = hipStreamSynchronize(e->stream);
-    void* t = tmp;
-    e->dfree(t, (size_t)std::max<int64_t>(1, e->n_atoms) * sizeof(double2));
+    e->dfree(vs, ns * sizeof(double2));
+    e->dfree(va, na * sizeof(double2));
     HIPX(ce);
     HIPX(se);
     e->kcollect();

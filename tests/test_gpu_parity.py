@@ -12,10 +12,11 @@ import zincblende as zb
 pytestmark = pytest.mark.gpu
 
 COMBOS = [(clat.PREC_FP64, clat.SCHED_SHARED), (clat.PREC_FP64, clat.SCHED_ROUNDS),
-          (clat.PREC_FP32, clat.SCHED_SHARED), (clat.PREC_FP32, clat.SCHED_ROUNDS)]
-IDS = ["fp64-shared", "fp64-rounds", "fp32-shared", "fp32-rounds"]
-# relative tolerance per precision: FP64 path 1e-10 (vs oracle), FP32 path 1e-5
-TOL = {clat.PREC_FP64: 1e-10, clat.PREC_FP32: 1e-5}
+          (clat.PREC_FP32, clat.SCHED_SHARED), (clat.PREC_FP32, clat.SCHED_ROUNDS),
+          (clat.PREC_FP32X, clat.SCHED_SHARED), (clat.PREC_FP32X, clat.SCHED_ROUNDS)]
+IDS = ["fp64-shared", "fp64-rounds", "fp32-shared", "fp32-rounds", "fp32x-shared", "fp32x-rounds"]
+# relative tolerance per precision: FP64 path 1e-10 (vs oracle), complex64 paths 1e-5
+TOL = {clat.PREC_FP64: 1e-10, clat.PREC_FP32: 1e-5, clat.PREC_FP32X: 1e-5}
 
 
 def _mock(G):
@@ -31,7 +32,7 @@ def test_two_point_charges(prec, sched):
     print("case two_point_charges", prec, sched, flush=True)
     cc = clat.CoulombCalculator((5, 5, 10), prec, sched)
     cc.initialize(clat.InteractionTable(_mock))
-    tol = 1e-12 if prec == clat.PREC_FP64 else 1e-6
+    tol = 1e-12 if prec == clat.PREC_FP64 else (1e-7 if prec == clat.PREC_FP32X else 1e-6)
     cc.input[...] = 0
     cc.input[2, 2, 2] = 1.0
     cc.prepare()
@@ -44,7 +45,8 @@ def test_two_point_charges(prec, sched):
 
 
 @pytest.mark.parametrize("prec,sched", COMBOS, ids=IDS)
-@pytest.mark.parametrize("dims", [(6, 5, 7), (16, 8, 4), (9, 10, 12), (11, 13, 5), (1, 4, 3)])
+@pytest.mark.parametrize("dims", [(6, 5, 7), (16, 8, 4), (9, 10, 12), (11, 13, 5), (1, 4, 3), (32, 20, 24),
+                                  (401, 3, 2)])
 def test_prepare_matches_oracle(prec, sched, dims):
     """Random complex input on odd, mixed-radix, non-smooth (11, 13) and
     degenerate grids: potential and calculate() vs the 8-round oracle."""
@@ -117,7 +119,7 @@ def test_zincblende_integrals(prec, sched):
     assert np.max(err) < TOL[prec], np.max(err)
 
 
-@pytest.mark.parametrize("prec", [clat.PREC_FP64, clat.PREC_FP32], ids=["fp64", "fp32"])
+@pytest.mark.parametrize("prec", [clat.PREC_FP64, clat.PREC_FP32, clat.PREC_FP32X], ids=["fp64", "fp32", "fp32x"])
 def test_potential_at_atoms(prec):
     """potentials.cpp:140-158: V of each self product at every atom's cell."""
     print("case potentials", prec, flush=True)
@@ -137,7 +139,7 @@ def test_potential_at_atoms(prec):
         assert np.max(np.abs(v - Vo)) / np.max(np.abs(Vo)) < TOL[prec]
 
 
-@pytest.mark.parametrize("prec", [clat.PREC_FP64, clat.PREC_FP32], ids=["fp64", "fp32"])
+@pytest.mark.parametrize("prec", [clat.PREC_FP64, clat.PREC_FP32, clat.PREC_FP32X], ids=["fp64", "fp32", "fp32x"])
 def test_thomas_fermi(prec):
     """InteractionThomasFermi (Interaction.cpp:18-39) table vs the oracle's."""
     print("case thomas_fermi", prec, flush=True)
@@ -156,3 +158,28 @@ def test_thomas_fermi(prec):
     scale = np.max(np.abs(want))
     err = np.abs(got - want) / np.maximum(np.abs(want), 1e-3 * scale)
     assert np.max(err) < TOL[prec]
+
+
+@pytest.mark.parametrize("prec,sched", COMBOS, ids=IDS)
+def test_golden_direct_sums(prec, sched, golden_dir):
+    """Engine vs the committed direct O(N^2) lattice-sum fixtures (no FFT in
+    the checker), including onsite != 0 and the 4096-atom 32^3 case."""
+    import json
+    with open(os.path.join(golden_dir, "zincblende_direct.json")) as f:
+        cases = json.load(f)["cases"]
+    for case in cases:
+        print("case golden", case["name"], prec, sched, flush=True)
+        w = zb.workload(*case["cells"], case["holes"], case["electrons"], config=case["config"],
+                        sigma_frac=case["sigma_frac"], n_orb=case["orbitals"])
+        X, Y, Z = w["dims"]
+        eng = clat.Engine((X, Y, Z), prec, sched)
+        eng.initialize(clat.InteractionSimple(w["step"], case["onsite"], case["dielectric"]))
+        eng.set_atoms(w["cells"], case["orbitals"])
+        for s in w["states"]:
+            eng.add_state(s)
+        got = eng.compute_integrals(case["quads"])
+        want = np.array([complex(a, b) for a, b in case["values"]])
+        scale = np.max(np.abs(want))
+        err = np.abs(got - want) / np.maximum(np.abs(want), 1e-3 * scale)
+        assert np.max(err) < TOL[prec], (case["name"], np.max(err))
+        eng.close()
