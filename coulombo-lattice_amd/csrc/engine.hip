@@ -212,6 +212,7 @@ struct clat_engine {
   int Xi = 0, Yi = 0, Zi = 0;  // engine grid
   int prec = 0, sched = 0;     // requested precision and schedule
   int rprec = 0;               // precision the passes run in
+  int mix = 0;                 // mixed mode: passes computing in FP64 (0: last only .. 3: all)
   bool fast = false;           // two-level pass kernels on all three axes
   int device = 0;
   hipStream_t stream = nullptr;
@@ -398,10 +399,12 @@ P2Args base_args(const clat_engine* e) {
 
 // One prepare() on the two-level kernels.
 //   TIN: dense input type, TA: x-forward output, TB: y-forward / conv output,
-//   TD: y-inverse output, TV: dense potential type, TC: arithmetic.
+//   TD: y-inverse output, TV: dense potential type, TC: arithmetic of every
+//   pass but the last, TCX: arithmetic of the last (inverse x) pass.
 // sites_in / sites_out: pair density at the sites and potential at the sites
 // (atom path); when null, the dense F / V grids are used.
-template <typename TIN, typename TA, typename TB, typename TD, typename TV, typename TC>
+template <typename TIN, typename TA, typename TB, typename TD, typename TV, typename TC, typename TCX,
+          typename TCZ = TC, typename TCY = TC>
 void prepare_fast(clat_engine* e, const double2* sites_in, double2* sites_out) {
   const int Xi = e->Xi, Yi = e->Yi, Zi = e->Zi;
   const double N = (double)e->N();
@@ -445,10 +448,10 @@ void prepare_fast(clat_engine* e, const double2* sites_in, double2* sites_out) {
       a.site_cell = e->d_site_cell;
       a.site_out = sites_out;
       const double bytes = sd * N * nsub + site_bytes + (acc ? 16.0 * e->n_sites : 0.0);
-      launch2<TD, TC, double, PM_INV, true>(e, e->cxi_s, a, 1, 0, in, nullptr, K_XINV, bytes);
+      launch2<TD, TCX, double, PM_INV, true>(e, e->cxi_s, a, 1, 0, in, nullptr, K_XINV, bytes);
     } else {
       const double bytes = sd * N * nsub + sv * N * (acc ? 2 : 1);
-      launch2<TD, TC, TV, PM_INV, true>(e, e->cxi, a, 1, 0, in, e->V, K_XINV, bytes);
+      launch2<TD, TCX, TV, PM_INV, true>(e, e->cxi, a, 1, 0, in, e->V, K_XINV, bytes);
     }
   };
 
@@ -466,14 +469,14 @@ void prepare_fast(clat_engine* e, const double2* sites_in, double2* sites_out) {
       a.L = Zi; a.nsub = 2; a.W = (int)W;
       a.in_os = a.out_os = W; a.in_is = a.out_is = 2LL * Yi * W;
       a.gxm = 1; a.gym = 1;
-      launch2<TB, TC, TB, PM_CONV, false>(e, e->czc, a, 2 * Yi, 2, e->B, e->B, K_ZCONV,
-                                          sb * 8 * N + (double)sizeof(TC) * (double)e->NG());
+      launch2<TB, TCZ, TB, PM_CONV, false>(e, e->czc, a, 2 * Yi, 2, e->B, e->B, K_ZCONV,
+                                           sb * 8 * N + (double)sizeof(TCZ) * (double)e->NG());
     }
     {
       P2Args a = base_args(e);
       a.L = Yi; a.nsub = 2; a.W = (int)W;
       a.in_os = 2LL * Yi * W; a.in_is = W; a.out_os = (long long)Yi * W; a.out_is = W;
-      launch2<TB, TC, TD, PM_INV, false>(e, e->cyi, a, Zi, 1, e->B, e->D, K_YINV, (sb * 4 + sd * 2) * N);
+      launch2<TB, TCY, TD, PM_INV, false>(e, e->cyi, a, Zi, 1, e->B, e->D, K_YINV, (sb * 4 + sd * 2) * N);
     }
     xinv(2, 0, 0, 0, 0, e->D);
   } else {
@@ -489,9 +492,9 @@ void prepare_fast(clat_engine* e, const double2* sites_in, double2* sites_out) {
       c.L = Zi; c.nsub = 1; c.W = (int)W;
       c.in_os = c.out_os = W; c.in_is = c.out_is = (long long)Yi * W;
       c.zpar = kz; c.gxm = 2; c.gxa = kx; c.gym = 2; c.gya = ky;
-      launch2<TA, TC, TA, PM_CONV, false>(e, e->czc, c, Yi, 2, e->A, e->A, K_ZCONV,
-                                          2 * sa * N + (double)sizeof(TC) * (double)e->NG());
-      launch2<TA, TC, TD, PM_INV, false>(e, e->cyi, a, Zi, 1, e->A, e->D, K_YINV, (sa + sd) * N);
+      launch2<TA, TCZ, TA, PM_CONV, false>(e, e->czc, c, Yi, 2, e->A, e->A, K_ZCONV,
+                                           2 * sa * N + (double)sizeof(TCZ) * (double)e->NG());
+      launch2<TA, TCY, TD, PM_INV, false>(e, e->cyi, a, Zi, 1, e->A, e->D, K_YINV, (sa + sd) * N);
       xinv(1, kx, ky, kz, round > 0 ? 1 : 0, e->D);
     }
   }
@@ -641,13 +644,24 @@ void ensure_dense(clat_engine* e) {
   if (!e->V) e->V = e->dalloc(e->v_bytes);
 }
 
+// The mixed mode: complex64 storage; FP64 arithmetic in the last pass only
+// (mix 0) or also in the y-inverse (1), the conv (2) and the forward (3) passes.
+void fp32x_prepare(clat_engine* e, const double2* si, double2* so) {
+  switch (e->mix) {
+    case 0: prepare_fast<double, float, float, float, double, float, double, float, float>(e, si, so); break;
+    case 1: prepare_fast<double, float, float, double, double, float, double, float, double>(e, si, so); break;
+    case 2: prepare_fast<double, float, float, double, double, float, double, double, double>(e, si, so); break;
+    default: prepare_fast<double, float, float, double, double, double, double, double, double>(e, si, so); break;
+  }
+}
+
 // dense F -> dense V
 void prepare_dense(clat_engine* e) {
   ensure_dense(e);
   if (e->fast) {
-    if (e->rprec == CLAT_PREC_FP64) prepare_fast<double, double, double, double, double, double>(e, nullptr, nullptr);
-    else if (e->rprec == CLAT_PREC_FP32) prepare_fast<float, float, float, float, float, float>(e, nullptr, nullptr);
-    else prepare_fast<double, float, float, double, double, double>(e, nullptr, nullptr);
+    if (e->rprec == CLAT_PREC_FP64) prepare_fast<double, double, double, double, double, double, double>(e, nullptr, nullptr);
+    else if (e->rprec == CLAT_PREC_FP32) prepare_fast<float, float, float, float, float, float, float>(e, nullptr, nullptr);
+    else fp32x_prepare(e, nullptr, nullptr);
   } else {
     if (e->rprec == CLAT_PREC_FP32) prepare_generic<float, 32>(e);
     else prepare_generic<double, 16>(e);
@@ -657,9 +671,9 @@ void prepare_dense(clat_engine* e) {
 // pair density at the sites -> potential at the sites (FP64)
 void prepare_sites(clat_engine* e, const double2* rho_sites, double2* v_sites) {
   if (e->fast) {
-    if (e->rprec == CLAT_PREC_FP64) prepare_fast<double, double, double, double, double, double>(e, rho_sites, v_sites);
-    else if (e->rprec == CLAT_PREC_FP32) prepare_fast<float, float, float, float, float, float>(e, rho_sites, v_sites);
-    else prepare_fast<double, float, float, double, double, double>(e, rho_sites, v_sites);
+    if (e->rprec == CLAT_PREC_FP64) prepare_fast<double, double, double, double, double, double, double>(e, rho_sites, v_sites);
+    else if (e->rprec == CLAT_PREC_FP32) prepare_fast<float, float, float, float, float, float, float>(e, rho_sites, v_sites);
+    else fp32x_prepare(e, rho_sites, v_sites);
     return;
   }
   ensure_dense(e);
@@ -782,10 +796,11 @@ void build_products(clat_engine* e) {
   if (e->n_sites == 0) return;
   const size_t per_site = (size_t)e->n_states * e->n_orb * sizeof(double2);
   int A = (int)std::min<size_t>(64, 49152 / per_site);
+  while (A > 1 && (size_t)(A + 1) * per_site > 49152) --A;
   e->kbegin();
   if (A >= 1) {
     const unsigned nb = (unsigned)((e->n_sites + A - 1) / A);
-    hipLaunchKernelGGL(k_products, dim3(nb), dim3(NT), (size_t)A * per_site, e->stream, e->coeffs,
+    hipLaunchKernelGGL(k_products, dim3(nb), dim3(NT), (size_t)(A + 1) * per_site, e->stream, e->coeffs,
                        e->d_site_atom, e->n_sites, e->n_atoms, e->n_states, e->n_orb, A, e->rho);
   } else {
     hipLaunchKernelGGL(k_products_direct, dim3(grid1d((size_t)P * e->n_sites)), dim3(256), 0, e->stream,
@@ -936,9 +951,14 @@ int clat_create(int X, int Y, int Z, int precision, int schedule, clat_engine** 
     switch (e->rprec) {
       case CLAT_PREC_FP64: e->s_in = e->s_a = e->s_b = e->s_d = e->s_v = dbl; break;
       case CLAT_PREC_FP32: e->s_in = e->s_a = e->s_b = e->s_d = e->s_v = flt; break;
-      default: e->s_in = dbl; e->s_a = flt; e->s_b = flt; e->s_d = dbl; e->s_v = dbl; break;
+      default: e->s_in = dbl; e->s_a = flt; e->s_b = flt; e->s_d = flt; e->s_v = dbl; break;
     }
-    const size_t tc = e->rprec == CLAT_PREC_FP32 ? flt : dbl;  // arithmetic type
+    if (const char* m = std::getenv("CLAT_MIX")) e->mix = std::max(0, std::min(3, std::atoi(m)));
+    if (e->rprec == CLAT_PREC_FP32X && e->mix >= 1) e->s_d = dbl;
+    const size_t tc = (e->rprec == CLAT_PREC_FP64 || (e->rprec == CLAT_PREC_FP32X && e->mix >= 3)) ? dbl : flt;
+    const size_t tcz = (e->rprec == CLAT_PREC_FP64 || (e->rprec == CLAT_PREC_FP32X && e->mix >= 2)) ? dbl : flt;
+    const size_t tcy = (e->rprec == CLAT_PREC_FP64 || (e->rprec == CLAT_PREC_FP32X && e->mix >= 1)) ? dbl : flt;
+    const size_t tcx = e->rprec == CLAT_PREC_FP32 ? flt : dbl;   // arithmetic of the inverse x pass
     const int dims[3] = {e->Xi, e->Yi, e->Zi};
     AxisPlan* lp[3] = {&e->px, &e->py, &e->pz};
     AxisPlan* dp[3] = {&e->dx, &e->dy, &e->dz};
@@ -954,11 +974,17 @@ int clat_create(int X, int Y, int Z, int precision, int schedule, clat_engine** 
       const int W = nsub * e->Xi;
       e->cxf = cfg_line(e->Xi, nsub, tc, false, e->lds_budget);
       e->cxf_s = cfg_line(e->Xi, nsub, tc, true, e->lds_budget);
-      e->cxi = cfg_line(e->Xi, nsub, tc, false, e->lds_budget);
-      e->cxi_s = cfg_line(e->Xi, nsub, tc, true, e->lds_budget);
+      e->cxi = cfg_line(e->Xi, nsub, tcx, false, e->lds_budget);
+      e->cxi_s = cfg_line(e->Xi, nsub, tcx, true, e->lds_budget);
       e->cyf = cfg_col(e->Yi, nsub, tc, W, e->lds_budget);
-      e->czc = cfg_col(e->Zi, nsub, tc, W, e->lds_budget);
-      e->cyi = cfg_col(e->Yi, nsub, tc, W, e->lds_budget);
+      // the complex64 conv pass runs best with half-size tiles (more blocks in flight)
+      size_t zb = tcz == flt ? e->lds_budget / 2 : e->lds_budget;
+      if (const char* zl = std::getenv("CLAT_LDS_CONV")) {
+        const long v = std::atol(zl);
+        if (v >= 4096 && v <= 65536) zb = (size_t)v;
+      }
+      e->czc = cfg_col(e->Zi, nsub, tcz, W, zb);
+      e->cyi = cfg_col(e->Yi, nsub, tcy, W, e->lds_budget);
       const PassCfg* all[] = {&e->cxf, &e->cxf_s, &e->cxi, &e->cxi_s, &e->cyf, &e->czc, &e->cyi};
       for (const PassCfg* p : all)
         if (p->lds > 65536) throw host::Error(CLAT_EUNSUP, "pass tile exceeds the LDS budget");
@@ -983,7 +1009,7 @@ int clat_create(int X, int Y, int Z, int precision, int schedule, clat_engine** 
       HIPX(hipMemcpy(e->dtw[d], dt.data(), dt.size() * sizeof(double), hipMemcpyHostToDevice));
     }
     e->G64 = (double*)e->dalloc(e->NG() * sizeof(double));
-    if (e->rprec == CLAT_PREC_FP32) e->G32 = (float*)e->dalloc(e->NG() * sizeof(float));
+    if (tcz == flt) e->G32 = (float*)e->dalloc(e->NG() * sizeof(float));
     const size_t N = e->N();
     e->f_bytes = N * e->s_in;
     e->v_bytes = N * e->s_v;
@@ -1242,6 +1268,7 @@ int clat_compute_integrals(clat_engine* e, const int32_t* quads, int64_t n, int 
       }
     const int nL = (int)slots.size();
     HIPX(hipEventRecord(e->stage_ev[1], e->stream));
+    e->rho_states = -1;  // the pair densities are part of every run's compute stage
     build_products(e);
     HIPX(hipEventRecord(e->stage_ev[2], e->stream));
     const size_t vat_need = (size_t)std::max(1, nL) * (size_t)std::max<int64_t>(1, e->n_sites);
@@ -1278,8 +1305,8 @@ int clat_compute_integrals(clat_engine* e, const int32_t* quads, int64_t n, int 
       hipLaunchKernelGGL(k_overlap_partial, dim3(tl, tr, (unsigned)n_chunks), dim3(NT), 0, e->stream, e->Vat,
                          e->rho, nL, P, e->n_sites, chunk, e->part);
       LAUNCH_CHECK();
-      hipLaunchKernelGGL(k_overlap_reduce, dim3(grid1d(n_out)), dim3(256), 0, e->stream, e->part, n_chunks,
-                         (int64_t)n_out, e->ovl);
+      hipLaunchKernelGGL(k_overlap_reduce, dim3((unsigned)((n_out + 63) / 64)), dim3(1024), 0, e->stream, e->part,
+                         n_chunks, (int64_t)n_out, e->ovl);
       LAUNCH_CHECK();
       e->kend(K_OVERLAP, (double)(tl * tr) * (double)(2 * OV_T) * (double)e->n_sites * 16.0);
       HIPX(hipMemcpyAsync(ov.data(), e->ovl, n_out * sizeof(double), hipMemcpyDeviceToHost, e->stream));

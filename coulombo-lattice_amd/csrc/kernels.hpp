@@ -413,19 +413,21 @@ __global__ __launch_bounds__(NT) void k_products(const double2* __restrict__ coe
                                                  int64_t n_atoms, int n_states, int n_orb, int A,
                                                  double2* __restrict__ rho) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
-  double2* cs = reinterpret_cast<double2*>(smem_raw);  // [s][a][o]
+  double2* cs = reinterpret_cast<double2*>(smem_raw);  // [s][o][a], pitch AP per orbital row
+  const int AP = A + 1;
   const int64_t k0 = (int64_t)blockIdx.x * A;
   const int nA = (int)min((int64_t)A, n_sites - k0);
   const int per = A * n_orb;
   const int tot = n_states * per;
   for (int e = threadIdx.x; e < tot; e += NT) {
+    // global order: state, site, orbital (orbital fastest: contiguous rows)
     const int s = e / per;
     const int r = e - s * per;
     const int a = r / n_orb;
     const int o = r - a * n_orb;
     double2 v = make_double2(0.0, 0.0);
     if (a < nA) v = coeffs[((size_t)s * n_atoms + atom[k0 + a]) * n_orb + o];
-    cs[e] = v;
+    cs[((size_t)s * n_orb + o) * AP + a] = v;
   }
   __syncthreads();
   const int P = n_states * (n_states + 1) / 2;
@@ -439,11 +441,11 @@ __global__ __launch_bounds__(NT) void k_products(const double2* __restrict__ coe
     while (fL * (fL + 1) / 2 > pidx) --fL;
     while ((fL + 1) * (fL + 2) / 2 <= pidx) ++fL;
     const int fR = pidx - fL * (fL + 1) / 2;
-    const double2* l = cs + (size_t)fL * per + a * n_orb;
-    const double2* r = cs + (size_t)fR * per + a * n_orb;
+    const double2* l = cs + (size_t)fL * n_orb * AP + a;
+    const double2* r = cs + (size_t)fR * n_orb * AP + a;
     double re = 0.0, im = 0.0;
     for (int o = 0; o < n_orb; ++o) {
-      const double2 x = l[o], y = r[o];
+      const double2 x = l[(size_t)o * AP], y = r[(size_t)o * AP];
       re += x.x * y.x + x.y * y.y;
       im += x.x * y.y - x.y * y.x;
     }
@@ -544,18 +546,33 @@ __global__ __launch_bounds__(NT) void k_overlap_partial(const double2* __restric
     }
 }
 
-// Stage 2: compensated (Neumaier) sum of the chunk partials.
-__global__ void k_overlap_reduce(const double* __restrict__ part, int64_t n_chunks, int64_t n_out,
-                                 double* __restrict__ out) {
-  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n_out; t += (int64_t)gridDim.x * blockDim.x) {
-    double s = 0.0, comp = 0.0;
-    for (int64_t c = 0; c < n_chunks; ++c) {
-      const double x = part[(size_t)c * n_out + t];
-      const double u = s + x;
-      comp += (fabs(s) >= fabs(x)) ? (s - u) + x : (x - u) + s;
-      s = u;
+// Stage 2: compensated sum of the chunk partials.  A block owns 64 outputs;
+// its 16 row groups of lanes each run a Neumaier sum over a stride of the
+// chunks, then the 16 (sum, compensation) pairs are merged in LDS.
+__device__ __forceinline__ void neumaier_add(double& s, double& c, double x) {
+  const double t = s + x;
+  c += (fabs(s) >= fabs(x)) ? (s - t) + x : (x - t) + s;
+  s = t;
+}
+
+__global__ __launch_bounds__(1024) void k_overlap_reduce(const double* __restrict__ part, int64_t n_chunks,
+                                                         int64_t n_out, double* __restrict__ out) {
+  __shared__ double ss[16][64], sc[16][64];
+  const int lane = threadIdx.x & 63, grp = threadIdx.x >> 6;
+  const int64_t o = (int64_t)blockIdx.x * 64 + lane;
+  double s = 0.0, c = 0.0;
+  if (o < n_out)
+    for (int64_t ch = grp; ch < n_chunks; ch += 16) neumaier_add(s, c, part[(size_t)ch * n_out + o]);
+  ss[grp][lane] = s;
+  sc[grp][lane] = c;
+  __syncthreads();
+  if (grp == 0 && o < n_out) {
+    double S = 0.0, Cc = 0.0;
+    for (int g = 0; g < 16; ++g) {
+      neumaier_add(S, Cc, ss[g][lane]);
+      Cc += sc[g][lane];
     }
-    out[t] = s + comp;
+    out[o] = S + Cc;
   }
 }
 

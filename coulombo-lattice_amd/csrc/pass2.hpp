@@ -78,7 +78,7 @@ __global__ __launch_bounds__(256) void k_pass2(const C2<TI>* in, C2<TO>* out, P2
   const CT zero = mk<TC>((TC)0, (TC)0);
 
   auto lidx = [&](int c, int s, int k2, int n1) -> int {
-    return LINE ? ((c * nsub + s) * R2 + k2) * a.R1p + n1 : ((s * R2 + k2) * R1 + n1) * C + c;
+    return LINE ? ((c * nsub + s) * R2 + k2) * a.R1p + n1 : ((((s * R2 + k2) * R1 + n1)) << lgC) + c;
   };
   auto gin = [&](int c, int i) -> size_t {
     return LINE ? (size_t)(base + c) * a.in_os + (size_t)i * a.in_is
@@ -368,7 +368,7 @@ hipError_t p2_launch(int combo, int mode, bool line, const P2Launch& L) {
       if (mode == PM_CONV && !line) return p2_go<f, d, f, PM_CONV, false, R1T, R2T>(L);
       break;
     case CB_FDD:
-      if (mode == PM_INV && !line) return p2_go<f, d, d, PM_INV, false, R1T, R2T>(L);
+      if (mode == PM_INV) return line ? p2_go<f, d, d, PM_INV, true, R1T, R2T>(L) : p2_go<f, d, d, PM_INV, false, R1T, R2T>(L);
       break;
     default:
       break;

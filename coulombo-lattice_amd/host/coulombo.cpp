@@ -37,7 +37,8 @@ const char* kUsage =
     "  --threads-per-node=N   number of host threads for reading input files, default: all (max 16)\n"
     "  --tf-lattice=VALUE     lattice constant (A) for Thomas-Fermi-Resta model,\n"
     "                           default: model not applied\n"
-    "  --precision=P          fp64 (default) or fp32: complex64 FFT passes with FP64 table and sums\n"
+    "  --precision=P          fp64 (default); fp32x: complex64 passes, FP64 last pass and sums\n"
+    "                           (every integral within 1e-5 of fp64); fp32: complex64 throughout\n"
     "  --schedule=S           shared (default): the 8 rounds share x/y passes; rounds: 8 full rounds\n"
     "\nFLAGS:\n"
     "  --reference-stale-values  reuse the previous overlap when only the left product changes,\n"
@@ -78,6 +79,7 @@ void run(const cli::Options& o) {
   if (o.has("precision", value)) {
     if (value == "fp64") precision = CLAT_PREC_FP64;
     else if (value == "fp32") precision = CLAT_PREC_FP32;
+    else if (value == "fp32x") precision = CLAT_PREC_FP32X;
     else cli::failf("invalid value for precision: %s", value.c_str());
   }
   if (o.has("schedule", value)) {

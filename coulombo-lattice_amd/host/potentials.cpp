@@ -32,7 +32,7 @@ const char* kUsage =
     "  --threads-per-node=N   number of host threads for reading input files, default: all (max 16)\n"
     "  --tf-lattice=VALUE     lattice constant (A) for Thomas-Fermi-Resta model,\n"
     "                           default: model not applied\n"
-    "  --precision=P          fp64 (default) or fp32\n"
+    "  --precision=P          fp64 (default), fp32x (complex64 passes, FP64 last pass) or fp32\n"
     "\n";
 
 int parse_positive(const cli::Options& o, const char* name, int def, const char* err) {
@@ -66,6 +66,7 @@ void run(const cli::Options& o) {
   if (o.has("precision", value)) {
     if (value == "fp64") precision = CLAT_PREC_FP64;
     else if (value == "fp32") precision = CLAT_PREC_FP32;
+    else if (value == "fp32x") precision = CLAT_PREC_FP32X;
     else cli::failf("invalid value for precision: %s", value.c_str());
   }
 

@@ -36,11 +36,11 @@ extern "C" {
  * overlap sum are FP64 in both modes. */
 #define CLAT_PREC_FP64 0  /* complex128 passes, phases and kernel multiply */
 #define CLAT_PREC_FP32 1  /* complex64 passes, phases and kernel multiply  */
-#define CLAT_PREC_FP32X 2 /* complex64 storage of the passes before and at the
-                             kernel multiply, FP64 arithmetic inside every pass,
-                             complex128 storage after the multiply: the
-                             complex64-bandwidth path that keeps every
-                             integral within 1e-5 of FP64                 */
+#define CLAT_PREC_FP32X 2 /* complex64 storage and arithmetic in every pass
+                             but the last; the last (inverse x) pass computes
+                             in FP64 and writes the potential in FP64, which
+                             keeps every integral within 1e-5 of FP64 at
+                             complex64 traffic                              */
 
 /* Schedule of the eight half-shift rounds (CoulombCalculator.cpp:105-163). */
 #define CLAT_SCHED_SHARED 0  /* rounds share their x and y passes: the eight

@@ -32,7 +32,7 @@ def test_two_point_charges(prec, sched):
     print("case two_point_charges", prec, sched, flush=True)
     cc = clat.CoulombCalculator((5, 5, 10), prec, sched)
     cc.initialize(clat.InteractionTable(_mock))
-    tol = 1e-12 if prec == clat.PREC_FP64 else (1e-7 if prec == clat.PREC_FP32X else 1e-6)
+    tol = 1e-12 if prec == clat.PREC_FP64 else 1e-6
     cc.input[...] = 0
     cc.input[2, 2, 2] = 1.0
     cc.prepare()
