@@ -67,6 +67,19 @@ def cpu_baseline(w, quads, budget_s=20.0):
                       "overlaps, extrapolated to the %d-integral run" % (len(lefts), t_prep, n_right, len(quads))}
 
 
+def traffic_of(kind):
+    """HBM bytes per launch of a pass kind from the committed rocprofv3 counter
+    summary (FETCH_SIZE + WRITE_SIZE, scaled by the plain-copy calibration),
+    or None when no summary exists for it."""
+    path = os.path.join(ROOT, "profiles", "traffic_1M_atoms.json")
+    try:
+        with open(path) as f:
+            t = json.load(f)
+        return t["kernels"][kind]["hbm_bytes_per_launch"]
+    except (OSError, KeyError, ValueError):
+        return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -77,7 +90,19 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--schedule", choices=["shared", "rounds"], default="shared")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only run a plain device-to-device copy of MIB MiB (counter calibration)")
     args = ap.parse_args()
+    if args.copy_probe:
+        import torch
+        n = args.copy_probe * (1 << 20) // 4
+        a = torch.ones(n, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        log("copy probe: 3 copies of %d MiB" % args.copy_probe)
+        return
     if args.gpus != 1:
         print("bench.py: this engine runs on one GPU; --gpus %d is not supported "
               "(the integral path needs no collectives: see DESIGN.md)" % args.gpus, file=sys.stderr)
@@ -161,7 +186,7 @@ def main():
                                "eeee,hhhh,ehhe,eheh" % (nx, ny, nz, len(w["cells"]), X, Y, Z, nh + ne),
                    "integrals": len(quads), "schedule": args.schedule},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                     "frac": achieved / PEAK_HBM_GBS, "traffic": None, "avg_launch_ms": avg_ms,
+                     "frac": achieved / PEAK_HBM_GBS, "traffic": traffic_of(dom), "avg_launch_ms": avg_ms,
                      "algorithmic_bytes_per_launch": byts / max(1, n_l)},
         "kernel_ms_per_step": {k: v[1] / args.steps for k, v in stats.items() if v[0]},
         "fft_pass_gbs": {k: (v[2] / max(1e-12, v[1] * 1e-3)) / 1e9 for k, v in stats.items()

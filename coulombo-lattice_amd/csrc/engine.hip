@@ -226,8 +226,11 @@ struct clat_engine {
   // device tables
   void *tw32[3] = {}, *h32[3] = {}, *tw64[3] = {}, *h64[3] = {};
   double2* dtw[3] = {};
-  double* G64 = nullptr;
-  float* G32 = nullptr;
+  double* G64 = nullptr;  // Ghat octant (Z+1)(Y+1)(X+1), FP64
+  float* G32 = nullptr;   // same, FP32 (generic complex64 kernels)
+  double* Gc64 = nullptr; // Ghat as [r(fy)][r(fz)][fx], fx in [0, 2X): conv pass tables
+  float* Gc32 = nullptr;
+  size_t gc_elems = 0;
   bool initialized = false;
   // grids
   void *F = nullptr, *V = nullptr;  // dense input / potential (grid protocol, generic path)
@@ -334,9 +337,15 @@ template <typename TC>
 const C2<TC>* h_of(const clat_engine* e, int axis) {
   return (const C2<TC>*)(sizeof(TC) == 8 ? e->h64[axis] : e->h32[axis]);
 }
+// octant table (generic kernels)
 template <typename TC>
 const TC* g_of(const clat_engine* e) {
   return (const TC*)(sizeof(TC) == 8 ? (const void*)e->G64 : (const void*)e->G32);
+}
+// conv-layout table [r(fy)][r(fz)][fx] of the two-level kernels
+template <typename TC>
+const TC* gc_of(const clat_engine* e) {
+  return (const TC*)(sizeof(TC) == 8 ? (const void*)e->Gc64 : (const void*)e->Gc32);
 }
 
 template <typename TI, typename TC, typename TO>
@@ -370,7 +379,7 @@ void launch2(clat_engine* e, const PassCfg& pc, P2Args a, int n_outer, int axis,
   L.h = h_of<TC>(e, axis);
   L.hy = h_of<TC>(e, 1);
   L.hz = h_of<TC>(e, 2);
-  L.G = g_of<TC>(e);
+  L.G = gc_of<TC>(e);
   constexpr int combo = combo_of<TI, TC, TO>();
   e->kbegin();
   hipError_t err;
@@ -391,8 +400,7 @@ P2Args base_args(const clat_engine* e) {
   a.X2 = 2 * e->Xi;
   a.Y2 = 2 * e->Yi;
   a.Z2 = 2 * e->Zi;
-  a.GX1 = e->Xi + 1;
-  a.GY1 = e->Yi + 1;
+  a.GZ1 = e->Zi + 1;
   a.Yl = e->Yi;
   return a;
 }
@@ -738,6 +746,16 @@ void finish_interaction(clat_engine* e) {
     hipLaunchKernelGGL(k_d2f, dim3(grid1d(e->NG())), dim3(256), 0, e->stream, e->G64, e->G32, e->NG());
     LAUNCH_CHECK();
   }
+  if (e->Gc64) {
+    hipLaunchKernelGGL(k_gconv<double>, dim3(grid1d(e->gc_elems)), dim3(256), 0, e->stream, e->G64, e->Gc64,
+                       e->Xi, e->Yi + 1, e->Zi + 1);
+    LAUNCH_CHECK();
+  }
+  if (e->Gc32) {
+    hipLaunchKernelGGL(k_gconv<float>, dim3(grid1d(e->gc_elems)), dim3(256), 0, e->stream, e->G64, e->Gc32,
+                       e->Xi, e->Yi + 1, e->Zi + 1);
+    LAUNCH_CHECK();
+  }
   e->initialized = true;
   e->prepared = false;
 }
@@ -1009,7 +1027,12 @@ int clat_create(int X, int Y, int Z, int precision, int schedule, clat_engine** 
       HIPX(hipMemcpy(e->dtw[d], dt.data(), dt.size() * sizeof(double), hipMemcpyHostToDevice));
     }
     e->G64 = (double*)e->dalloc(e->NG() * sizeof(double));
-    if (tcz == flt) e->G32 = (float*)e->dalloc(e->NG() * sizeof(float));
+    if (e->rprec == CLAT_PREC_FP32) e->G32 = (float*)e->dalloc(e->NG() * sizeof(float));
+    if (e->fast) {
+      e->gc_elems = (size_t)2 * e->Xi * (e->Yi + 1) * (e->Zi + 1);
+      if (tcz == flt) e->Gc32 = (float*)e->dalloc(e->gc_elems * sizeof(float));
+      else e->Gc64 = (double*)e->dalloc(e->gc_elems * sizeof(double));
+    }
     const size_t N = e->N();
     e->f_bytes = N * e->s_in;
     e->v_bytes = N * e->s_v;
@@ -1040,7 +1063,7 @@ int clat_destroy(clat_engine* e) {
   if (e->stream) (void)hipStreamSynchronize(e->stream);
   void* ptrs[] = {e->tw32[0], e->tw32[1], e->tw32[2], e->h32[0], e->h32[1], e->h32[2], e->tw64[0], e->tw64[1],
                   e->tw64[2], e->h64[0], e->h64[1], e->h64[2], e->dtw[0], e->dtw[1], e->dtw[2], e->G64,
-                  e->G32, e->F, e->V, e->A, e->B, (e->D != e->A) ? e->D : nullptr, e->Fin, e->dpart,
+                  e->G32, e->Gc64, e->Gc32, e->F, e->V, e->A, e->B, (e->D != e->A) ? e->D : nullptr, e->Fin, e->dpart,
                   e->d_site_atom, e->d_site_cell, e->d_site_ptr, e->d_atom_site, e->coeffs, e->rho, e->Vat,
                   e->part, e->ovl};
   for (void* p : ptrs)

@@ -347,6 +347,20 @@ __global__ void k_zero(C2<T>* __restrict__ p, size_t n) {
     p[t] = mk<T>((T)0, (T)0);
 }
 
+// Ĝ for the two-level conv pass: out[(ry*(Z1) + rz)*2X + fx] = Ghat[rz][ry][r(fx)]
+template <typename T>
+__global__ void k_gconv(const double* __restrict__ G, T* __restrict__ out, int X, int Y1, int Z1) {
+  const size_t n = (size_t)2 * X * Y1 * Z1;
+  for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < n; t += (size_t)gridDim.x * blockDim.x) {
+    const int fx = (int)(t % (2 * X));
+    const size_t q = t / (2 * X);
+    const int rz = (int)(q % Z1);
+    const int ry = (int)(q / Z1);
+    const int rx = min(fx, 2 * X - fx);
+    out[t] = (T)G[((size_t)rz * Y1 + ry) * (X + 1) + rx];
+  }
+}
+
 __global__ void k_d2f(const double* __restrict__ a, float* __restrict__ b, size_t n) {
   for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < n; t += (size_t)gridDim.x * blockDim.x)
     b[t] = (float)a[t];

@@ -32,8 +32,9 @@ struct P2Args {
   long long in_os, in_is, out_os, out_is;
   // rounds schedule phase of an x pass: (kx ? h[n] : 1) * hy[y]^ky * hz[z]^kz, line = y + Yl*z
   int rphase, kx, ky, kz, Yl;
-  // kernel-table lookup of the conv pass
-  int zpar, gxm, gxa, gym, gya, X2, Y2, Z2, GX1, GY1;
+  // kernel-table lookup of the conv pass: Ĝ stored as [r(fy)][r(fz)][fx],
+  // fx over the whole 2X range (x reflection folded in), pitch X2, rows GZ1
+  int zpar, gxm, gxa, gym, gya, X2, Y2, Z2, GZ1;
   int acc;           // inverse line pass: add to the output instead of writing it
   // atom sites of line mode: CSR over lines, internal linear cell of each site
   const int32_t* site_ptr;
@@ -213,14 +214,14 @@ __global__ __launch_bounds__(256) void k_pass2(const C2<TI>* in, C2<TO>* out, P2
       }
       if (MODE == PM_CONV) {
         const int fx = (base + min(c, nC - 1)) * a.gxm + a.gxa;
-        const int rx = min(fx, a.X2 - fx);
         const int zp = (nsub == 2) ? s : a.zpar;
+        const TC* Grow = G + (size_t)ry * a.GZ1 * a.X2 + fx;
 #pragma unroll
         for (int k1 = 0; k1 < M1; ++k1) {
           if (k1 < R1) {
             const int fz = 2 * (R2 * k1 + k2) + zp;
             const int rz = min(fz, a.Z2 - fz);
-            const TC g = G[((size_t)rz * a.GY1 + ry) * a.GX1 + rx];
+            const TC g = Grow[(size_t)rz * a.X2];
             v[k1] = cscale<TC>(v[k1], g);
           }
         }

@@ -1,0 +1,127 @@
+"""Condense rocprofv3 outputs of the integral workload into short text summaries.
+
+Reads the rocpd databases (``<dir>/**/*_results.db``) that
+
+    rocprofv3 --kernel-trace --stats -d <dir> -o run -- python3 bench.py ...
+    rocprofv3 --pmc FETCH_SIZE -d <dir> -o run -- python3 bench.py ...
+    rocprofv3 --pmc WRITE_SIZE -d <dir> -o run -- python3 bench.py ...
+
+write, plus the same two counter runs over ``bench.py --copy-probe MIB`` (a
+plain device copy of known size), and writes
+
+* a Markdown table of per-kernel call counts and average durations, and
+* ``--traffic-json``: per pass kind, FETCH_SIZE and WRITE_SIZE in KiB per
+  dispatch and the HBM bytes per launch after scaling each counter by
+  (bytes the copy moved) / (what the counter read for it).
+
+Pass kinds follow the kernel template arguments: mode 0/1/2 = forward / conv /
+inverse, line = x pass, column = y or z pass.
+"""
+import argparse
+import glob
+import json
+import os
+import re
+import sqlite3
+from collections import defaultdict
+
+
+def kind_of(name):
+    m = re.search(r"k_pass2<(\w+), (\w+), (\w+), (\d), (true|false), (\d+), (\d+)>", name)
+    if m:
+        mode, line = int(m.group(4)), m.group(5) == "true"
+        return {(0, True): "xfwd", (0, False): "yfwd", (1, False): "zconv",
+                (2, False): "yinv", (2, True): "xinv"}[(mode, line)]
+    for key in ("k_products_direct", "k_products", "k_overlap_partial", "k_overlap_reduce", "k_dct1",
+                "k_gconv", "k_gtable", "k_gpad", "k_d2f", "k_xfwd", "k_xinv", "k_col", "k_grid_dot",
+                "k_atoms_from_sites", "copyBuffer"):
+        if key in name:
+            return key[2:] if key.startswith("k_") else key
+    return name.split("(")[0][:60]
+
+
+def short(name):
+    m = re.search(r"(k_pass2<[^>]*>)", name)
+    return m.group(1) if m else name.split("(")[0].replace("void ", "")[:70]
+
+
+def db_of(d):
+    hits = sorted(glob.glob(os.path.join(d, "**", "*_results.db"), recursive=True))
+    if not hits:
+        raise SystemExit("no rocpd database under %s" % d)
+    return hits[0]
+
+
+def trace_stats(d):
+    con = sqlite3.connect(db_of(d))
+    rows = con.execute("select name, total_calls, total_duration, average from top_kernels").fetchall()
+    return [(n, int(c), float(t), float(a)) for n, c, t, a in rows]
+
+
+def counters(d, counter):
+    con = sqlite3.connect(db_of(d))
+    rows = con.execute("select kernel_name, counter_name, value from counters_collection").fetchall()
+    acc = defaultdict(list)
+    for k, c, v in rows:
+        if c == counter:
+            acc[k].append(float(v))
+    return {k: sum(v) / len(v) for k, v in acc.items()}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--trace", required=True, help="output dir of the --kernel-trace --stats run")
+    ap.add_argument("--fetch", help="output dir of the FETCH_SIZE run")
+    ap.add_argument("--write", help="output dir of the WRITE_SIZE run")
+    ap.add_argument("--copy-fetch", help="FETCH_SIZE run of bench.py --copy-probe")
+    ap.add_argument("--copy-write", help="WRITE_SIZE run of bench.py --copy-probe")
+    ap.add_argument("--copy-mib", type=float, default=1024.0)
+    ap.add_argument("--title", default="kernel trace")
+    ap.add_argument("--commands", default="", help="text block naming the commands that made the inputs")
+    ap.add_argument("--md", required=True)
+    ap.add_argument("--traffic-json")
+    a = ap.parse_args()
+
+    stats = trace_stats(a.trace)
+    total = sum(t for _, _, t, _ in stats)
+    out = ["# %s" % a.title, "", "Commands:", "", "```", a.commands.strip(), "```", "",
+           "| kernel | kind | calls | avg (us) | total (ms) | share |", "|---|---|---:|---:|---:|---:|"]
+    for n, c, t, avg in stats:
+        out.append("| `%s` | %s | %d | %.1f | %.2f | %.1f%% |" % (short(n), kind_of(n), c, avg, t / 1e3,
+                                                                   100.0 * t / total))
+    traffic = None
+    if a.fetch and a.write and a.copy_fetch and a.copy_write:
+        copy_bytes = a.copy_mib * (1 << 20)
+        cf = counters(a.copy_fetch, "FETCH_SIZE")
+        cw = counters(a.copy_write, "WRITE_SIZE")
+        ck = [k for k in cf if "copyBuffer" in k]
+        if not ck:
+            raise SystemExit("copy kernel not found in the calibration run")
+        f_factor = copy_bytes / (cf[ck[0]] * 1024.0)
+        w_factor = copy_bytes / (cw[ck[0]] * 1024.0)
+        fetch, write = counters(a.fetch, "FETCH_SIZE"), counters(a.write, "WRITE_SIZE")
+        per_kind = {}
+        for k in fetch:
+            kd = kind_of(k) if "k_pass2<" in k else short(k).replace("clat::", "")
+            fk, wk = fetch[k], write.get(k, 0.0)
+            per_kind[kd] = {"kernel": short(k), "fetch_kib": fk, "write_kib": wk,
+                            "hbm_bytes_per_launch": fk * 1024.0 * f_factor + wk * 1024.0 * w_factor}
+        traffic = {"copy_probe_mib": a.copy_mib, "fetch_factor": f_factor, "write_factor": w_factor,
+                   "kernels": per_kind}
+        out += ["", "## HBM traffic per dispatch", "",
+                "Copy calibration: %.0f MiB copy read as FETCH_SIZE %.0f KiB (factor %.3f), "
+                "WRITE_SIZE %.0f KiB (factor %.3f)." % (a.copy_mib, cf[ck[0]], f_factor, cw[ck[0]], w_factor),
+                "", "| kind | kernel | FETCH_SIZE KiB | WRITE_SIZE KiB | HBM MB (scaled) |",
+                "|---|---|---:|---:|---:|"]
+        for kd, v in sorted(per_kind.items()):
+            out.append("| %s | `%s` | %.0f | %.0f | %.1f |" % (kd, v["kernel"], v["fetch_kib"], v["write_kib"],
+                                                               v["hbm_bytes_per_launch"] / 1e6))
+    with open(a.md, "w") as f:
+        f.write("\n".join(out) + "\n")
+    if traffic is not None and a.traffic_json:
+        with open(a.traffic_json, "w") as f:
+            json.dump(traffic, f, indent=1, sort_keys=True)
+
+
+if __name__ == "__main__":
+    main()
