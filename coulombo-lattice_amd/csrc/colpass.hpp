@@ -1,0 +1,313 @@
+// Column passes (y forward, z convolution, y inverse) with every size fixed at
+// compile time: radix pair R1 x R2, sub-line count NSUB and the number of
+// columns per block C.  Compared with the general two-level kernel this one
+//   * keeps one sub-line in LDS at a time (the parities run one after the
+//     other), halving the LDS per block;
+//   * addresses LDS with compile-time offsets from one per-thread base, and
+//     global memory with 32-bit offsets from a per-block base pointer;
+//   * folds the half-shift phase h[n] = W_2L^n = W_2L^n1 W_2R2^n2 into the
+//     pass twiddle (W_2L^{n1 (2 k2 + s)}, one table of 2L roots) and a
+//     compile-time pre/post rotation by W_2R2^n2.
+//
+// Index map (L = R1 R2):  n = n1 + R1 n2,  k = R2 k1 + k2.
+// Thread t of the block owns column c = t % C and row r = t / C; r is n1 in
+// the phases that hold R2 values and k2 in the phase that holds R1 values.
+#pragma once
+
+#include "pass2.hpp"
+
+namespace clat {
+
+struct ColArgs2 {
+  int W;                          // columns in the plane
+  // outer index o = blockIdx.y (after the conv reordering) -> element offset
+  //   (o / od) * os_hi + (o % od) * os_lo
+  int od;
+  long long in_os_hi, in_os_lo, out_os_hi, out_os_lo;
+  // rows: n-domain row n1 + R1 n2 -> byte offset n2 * is_hi + n1 * is_lo;
+  // k-domain rows j (y passes only) -> j * is_lo
+  unsigned in_is_lo, in_is_hi, out_is_lo, out_is_hi;
+  // conv pass: fy = o * gym + gya; the kernel table is tiled per block of C
+  // columns as [ry][parity p][column block][rz][C]; pair = 1 orders the
+  // blocks so that fy and 2Y - fy (same table tile) run next to each other
+  int gym, gya, zpar, Y2, GZ1, ncb, npar, par, pair;
+};
+
+template <typename T>
+__device__ __forceinline__ T ldg_off(const T* base, uint32_t byte_off) {
+  return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + byte_off);
+}
+template <typename T>
+__device__ __forceinline__ void stg_off(T* base, uint32_t byte_off, T v) {
+  *reinterpret_cast<T*>(reinterpret_cast<char*>(base) + byte_off) = v;
+}
+
+// v * W_R^{e} (forward sign) or v * W_R^{-e} (CONJ), W_R = exp(-2 pi i / R), e compile-time
+template <int R, int E, bool CONJ, typename T>
+__device__ __forceinline__ C2<T> rot_const(C2<T> v) {
+  constexpr int e = ((E % R) + R) % R;
+  if constexpr (e == 0) {
+    return v;
+  } else if constexpr (2 * e == R) {
+    return mk<T>(-v.x, -v.y);
+  } else if constexpr (4 * e == R) {
+    // W^{R/4} = -i
+    return CONJ ? mk<T>(-v.y, v.x) : mk<T>(v.y, -v.x);
+  } else if constexpr (4 * e == 3 * R) {
+    return CONJ ? mk<T>(v.y, -v.x) : mk<T>(-v.y, v.x);
+  } else {
+    constexpr TwTable<R> tw{};
+    const T c = (T)tw.c[e];
+    const T s = CONJ ? (T)tw.s[e] : (T)(-tw.s[e]);
+    return mk<T>(v.x * c - v.y * s, v.x * s + v.y * c);
+  }
+}
+
+template <int R, bool CONJ, typename T, int I = 0>
+__device__ __forceinline__ void rot_all(C2<T>* v) {
+  // v[i] *= W_{2R}^{+-i}
+  if constexpr (I < R) {
+    v[I] = rot_const<2 * R, I, CONJ, T>(v[I]);
+    rot_all<R, CONJ, T, I + 1>(v);
+  }
+}
+
+// C: columns per block (power of two); NTH = C * R1 rounded up to a wave
+template <int C, int R1>
+constexpr int col_threads() {
+  return (C * R1 + 63) / 64 * 64;
+}
+
+template <typename TI, typename TC, typename TO, int MODE, int R1, int R2, int NSUB, int C>
+__global__ __launch_bounds__((col_threads<C, R1>())) void k_colpass(const C2<TI>* in,
+                                                                  C2<TO>* out, ColArgs2 a,
+                                                                  const C2<TC>* __restrict__ T2L,
+                                                                  const TC* __restrict__ G) {
+  using CT = C2<TC>;
+  constexpr int L = R1 * R2;
+  constexpr uint32_t SI = sizeof(C2<TI>), SO = sizeof(C2<TO>), SC = sizeof(CT);
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  CT* lds = reinterpret_cast<CT*>(smem_raw);
+  const int tid = threadIdx.x;
+  const int c = tid & (C - 1);
+  const int r = tid / C;
+  const int col = blockIdx.x * C + c;
+  const bool okc = col < a.W;
+  const CT zero = mk<TC>((TC)0, (TC)0);
+  // outer index; the conv pass pairs fy with its reflection 2Y - fy
+  int o = blockIdx.y;
+  if (MODE == PM_CONV && a.pair) {
+    const int j = o, last = (int)gridDim.y - 1;
+    const int t = (j + 1) >> 1;
+    o = (j == 0 || j == last) ? (j == 0 ? 0 : a.Y2 >> 1) : ((j & 1) ? t : a.Y2 - t);
+  }
+  const int oh = o / a.od, ol = o - oh * a.od;
+  // per-block bases; every later offset is 32-bit
+  const C2<TI>* bin = in + (size_t)oh * a.in_os_hi + (size_t)ol * a.in_os_lo + (size_t)blockIdx.x * C;
+  C2<TO>* bout = out + (size_t)oh * a.out_os_hi + (size_t)ol * a.out_os_lo + (size_t)blockIdx.x * C;
+  const uint32_t cin = (uint32_t)c * SI, cout = (uint32_t)c * SO;
+
+  // ------------------------------------------------------------------ load
+  // FWD / CONV: thread (c, n1 = r) loads x[n1 + R1 n2], n2 < R2
+  CT x[MODE == PM_INV ? 1 : R2];
+  if constexpr (MODE != PM_INV) {
+    const bool ok = okc && r < R1;
+    const uint32_t rbase = (uint32_t)r * a.in_is_lo + cin;
+#pragma unroll
+    for (int n2 = 0; n2 < R2; ++n2) {
+      const uint32_t off = rbase + (uint32_t)n2 * a.in_is_hi;
+      x[n2] = ok ? cvt<TI, TC>(ldg_off(bin, off)) : zero;
+    }
+  }
+  // INV: thread (c, k2 = r) loads X_s[R2 k1 + k2] of every sub-line up front
+  CT xi[MODE == PM_INV ? NSUB * R1 : 1];
+  if constexpr (MODE == PM_INV) {
+    const bool ok = okc && r < R2;
+#pragma unroll
+    for (int s = 0; s < NSUB; ++s)
+#pragma unroll
+      for (int k1 = 0; k1 < R1; ++k1) {
+        const uint32_t off = (uint32_t)((R2 * k1 + r) * NSUB + s) * a.in_is_lo + cin;
+        xi[s * R1 + k1] = ok ? cvt<TI, TC>(ldg_off(bin, off)) : zero;
+      }
+  }
+
+  CT acc[MODE == PM_FWD ? 1 : R2];
+  if constexpr (MODE != PM_FWD) {
+#pragma unroll
+    for (int n2 = 0; n2 < R2; ++n2) acc[n2] = zero;
+  }
+
+  // conv pass: the block's kernel-table tile, C values per rz row
+  const TC* Gb = nullptr;
+  if constexpr (MODE == PM_CONV) {
+    const int fy = o * a.gym + a.gya;
+    const int ry = min(fy, a.Y2 - fy);
+    Gb = G + ((size_t)(ry * a.npar + a.par) * a.ncb + blockIdx.x) * a.GZ1 * C + c;
+  }
+
+#pragma unroll
+  for (int s = 0; s < NSUB; ++s) {
+    // -------------------------------------------------------------- P1
+    if constexpr (MODE != PM_INV) {
+      if (r < R1) {
+        CT v[R2];
+#pragma unroll
+        for (int n2 = 0; n2 < R2; ++n2) v[n2] = x[n2];
+        if (s == 1) rot_all<R2, false, TC>(v);
+        rdft<R2, false, TC>(v);
+        const uint32_t tstep = (uint32_t)(2 * r) * SC;
+        const uint32_t tbase = (uint32_t)(s * r) * SC;
+#pragma unroll
+        for (int k2 = 0; k2 < R2; ++k2) {
+          CT t = v[k2];
+          if (k2 > 0 || s == 1) t = cmul<TC>(t, ldg_off(T2L, tbase + (uint32_t)k2 * tstep));
+          lds[(k2 * R1 + r) * C + c] = t;
+        }
+      }
+      __syncthreads();
+    }
+    // -------------------------------------------------------------- P2
+    if (r < R2) {
+      CT v[R1];
+      if constexpr (MODE == PM_INV) {
+#pragma unroll
+        for (int k1 = 0; k1 < R1; ++k1) v[k1] = xi[s * R1 + k1];
+      } else {
+#pragma unroll
+        for (int n1 = 0; n1 < R1; ++n1) v[n1] = lds[(r * R1 + n1) * C + c];
+        rdft<R1, false, TC>(v);  // v[k1] = X_s[R2 k1 + k2]
+      }
+      if constexpr (MODE == PM_FWD) {
+        if (okc) {
+#pragma unroll
+          for (int k1 = 0; k1 < R1; ++k1) {
+            const uint32_t off = (uint32_t)((R2 * k1 + r) * NSUB + s) * a.out_is_lo + cout;
+            stg_off(bout, off, cvt<TC, TO>(v[k1]));
+          }
+        }
+      } else {
+        if constexpr (MODE == PM_CONV) {
+          // fz = 2 (R2 k1 + k2) + parity, rz = min(fz, 2L - fz)
+          const int q = 2 * r + (NSUB == 2 ? s : a.zpar);
+          const uint32_t lo = (uint32_t)q * (C * sizeof(TC));
+#pragma unroll
+          for (int k1 = 0; k1 < R1; ++k1) {
+            // resolved per unrolled k1: rows below L, above L, or straddling it
+            constexpr uint32_t RB = C * sizeof(TC);
+            uint32_t off;
+            if (2 * R2 * (k1 + 1) - 1 <= L) off = lo + (uint32_t)(2 * R2 * k1) * RB;
+            else if (2 * R2 * k1 >= L) off = (uint32_t)(2 * L - 2 * R2 * k1) * RB - lo;
+            else {
+              const int fz = 2 * R2 * k1 + q;
+              off = (uint32_t)min(fz, 2 * L - fz) * RB;
+            }
+            v[k1] = cscale<TC>(v[k1], ldg_off(Gb, off));
+          }
+        }
+        rdft<R1, true, TC>(v);  // v[n1] = sum_k1 X W_R1^{-n1 k1}
+        const uint32_t tstep = (uint32_t)(2 * r + s) * SC;
+#pragma unroll
+        for (int n1 = 0; n1 < R1; ++n1) {
+          CT t = v[n1];
+          if (n1 > 0) t = cmulc<TC>(t, ldg_off(T2L, (uint32_t)n1 * tstep));
+          lds[(r * R1 + n1) * C + c] = t;
+        }
+      }
+    }
+    if constexpr (MODE != PM_FWD) {
+      __syncthreads();
+      // ------------------------------------------------------------ P3
+      if (r < R1) {
+        CT v[R2];
+#pragma unroll
+        for (int k2 = 0; k2 < R2; ++k2) v[k2] = lds[(k2 * R1 + r) * C + c];
+        rdft<R2, true, TC>(v);  // v[n2] = x_s[n1 + R1 n2] / W_2L^{n1 s}
+        if (s == 1) rot_all<R2, true, TC>(v);
+#pragma unroll
+        for (int n2 = 0; n2 < R2; ++n2) acc[n2] = cadd<TC>(acc[n2], v[n2]);
+      }
+    }
+    if (s + 1 < NSUB) __syncthreads();
+  }
+
+  // ------------------------------------------------------------------ store
+  if constexpr (MODE != PM_FWD) {
+    if (okc && r < R1) {
+      const uint32_t rbase = (uint32_t)r * a.out_is_lo + cout;
+#pragma unroll
+      for (int n2 = 0; n2 < R2; ++n2) {
+        const uint32_t off = rbase + (uint32_t)n2 * a.out_is_hi;
+        stg_off(bout, off, cvt<TC, TO>(acc[n2]));
+      }
+    }
+  }
+}
+
+// columns per block: 128-byte rows of the arithmetic type
+template <typename TC>
+constexpr int col_c() {
+  return sizeof(TC) == 4 ? 16 : 8;
+}
+
+struct ColLaunch {
+  dim3 grid;
+  hipStream_t stream;
+  const void* in;
+  void* out;
+  ColArgs2 a;
+  const void* T2L;
+  const void* G;
+};
+
+template <int R1, int R2, typename TC>
+constexpr size_t col_lds() {
+  return (size_t)R1 * R2 * col_c<TC>() * sizeof(C2<TC>);
+}
+
+template <typename TI, typename TC, typename TO, int MODE, int R1, int R2, int NSUB>
+hipError_t col_go(const ColLaunch& L) {
+  constexpr int C = col_c<TC>();
+  constexpr int nth = col_threads<C, R1>();
+  constexpr size_t lds = col_lds<R1, R2, TC>();
+  hipLaunchKernelGGL((k_colpass<TI, TC, TO, MODE, R1, R2, NSUB, C>), L.grid, dim3(nth), lds, L.stream, (const C2<TI>*)L.in, (C2<TO>*)L.out, L.a,
+                     (const C2<TC>*)L.T2L, (const TC*)L.G);
+  return hipGetLastError();
+}
+
+template <typename TI, typename TC, typename TO, int MODE, int R1, int R2>
+hipError_t col_go_n(int nsub, const ColLaunch& L) {
+  return nsub == 2 ? col_go<TI, TC, TO, MODE, R1, R2, 2>(L) : col_go<TI, TC, TO, MODE, R1, R2, 1>(L);
+}
+
+// (types, mode) pairs of the column passes; hipErrorInvalidValue otherwise
+template <int R1, int R2>
+hipError_t col_launch(int combo, int mode, int nsub, const ColLaunch& L) {
+  using d = double;
+  using f = float;
+  switch (combo) {
+    case CB_DDD:
+      if (mode == PM_FWD) return col_go_n<d, d, d, PM_FWD, R1, R2>(nsub, L);
+      if (mode == PM_CONV) return col_go_n<d, d, d, PM_CONV, R1, R2>(nsub, L);
+      return col_go_n<d, d, d, PM_INV, R1, R2>(nsub, L);
+    case CB_FFF:
+      if (mode == PM_FWD) return col_go_n<f, f, f, PM_FWD, R1, R2>(nsub, L);
+      if (mode == PM_CONV) return col_go_n<f, f, f, PM_CONV, R1, R2>(nsub, L);
+      return col_go_n<f, f, f, PM_INV, R1, R2>(nsub, L);
+    case CB_FDF:
+      if (mode == PM_FWD) return col_go_n<f, d, f, PM_FWD, R1, R2>(nsub, L);
+      if (mode == PM_CONV) return col_go_n<f, d, f, PM_CONV, R1, R2>(nsub, L);
+      break;
+    case CB_FDD:
+      if (mode == PM_INV) return col_go_n<f, d, d, PM_INV, R1, R2>(nsub, L);
+      break;
+    default:
+      break;
+  }
+  return hipErrorInvalidValue;
+}
+
+#define CLAT_C2_DECL(a, b) hipError_t clat_c2_##a##x##b(int combo, int mode, int nsub, const ColLaunch& L);
+CLAT_P2_PAIRS(CLAT_C2_DECL)
+
+}  // namespace clat

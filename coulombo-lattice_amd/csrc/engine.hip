@@ -13,7 +13,7 @@
 #include "../../include/clat.h"
 #include "host_logic.hpp"
 #include "kernels.hpp"
-#include "pass2.hpp"
+#include "colpass.hpp"
 
 using namespace clat;
 
@@ -225,7 +225,11 @@ struct clat_engine {
   size_t s_in = 16, s_a = 16, s_b = 16, s_d = 16, s_v = 16;
   // device tables
   void *tw32[3] = {}, *h32[3] = {}, *tw64[3] = {}, *h64[3] = {};
-  double2* dtw[3] = {};
+  double2* dtw[3] = {};         // W_2L^m, m < 2L, per axis (FP64)
+  float2* dtw32[3] = {};        // same, FP32 (column passes in complex64)
+  bool colk = true;             // compile-time column kernels for the y and z passes
+  bool conv_tiled = false;      // conv pass on the column kernel (tiled kernel table)
+  int gc_c = 16, gc_ncb = 1, gc_npar = 1, gc_w = 0;  // kernel-table tiling
   double* G64 = nullptr;  // Ghat octant (Z+1)(Y+1)(X+1), FP64
   float* G32 = nullptr;   // same, FP32 (generic complex64 kernels)
   double* Gc64 = nullptr; // Ghat as [r(fy)][r(fz)][fx], fx in [0, 2X): conv pass tables
@@ -356,9 +360,76 @@ constexpr int combo_of() {
        : CB_FDD;
 }
 
+bool col_pair_compiled(int R1, int R2) {
+  switch (R1 * 64 + R2) {
+#define CLAT_C2_HAVE(ra, rb) case ra * 64 + rb: return true;
+    CLAT_P2_PAIRS(CLAT_C2_HAVE)
+#undef CLAT_C2_HAVE
+    default: return false;
+  }
+}
+
+// row offsets of a column pass stay below 2^32 bytes
+bool col_fits(int L, int nsub, long long in_is, size_t si, long long out_is, size_t so) {
+  const double lim = 4294967295.0, rows = (double)L * nsub;
+  return rows * (double)in_is * (double)si + 16.0 * 16 < lim && rows * (double)out_is * (double)so + 16.0 * 16 < lim;
+}
+
+// Column pass on the compile-time column kernel; false when the radix pair
+// has no code object or an offset would not fit 32 bits (the general kernel
+// then runs it).  The conv pass follows the kernel-table layout chosen at
+// engine creation.
+template <typename TI, typename TC, typename TO, int MODE>
+bool launch_col(clat_engine* e, const PassCfg& pc, const P2Args& a, int n_outer, int axis, const void* in,
+                void* out, int kind, double bytes) {
+  if (MODE == PM_CONV) {
+    if (!e->conv_tiled) return false;
+  } else if (!e->colk || !col_pair_compiled(pc.R1, pc.R2) ||
+             !col_fits(a.L, a.nsub, a.in_is, sizeof(C2<TI>), a.out_is, sizeof(C2<TO>))) {
+    return false;
+  }
+  const int C = col_c<TC>();
+  ColLaunch L;
+  L.grid = dim3((unsigned)((a.W + C - 1) / C), (unsigned)n_outer);
+  L.stream = e->stream;
+  L.in = in;
+  L.out = out;
+  ColArgs2& c = L.a;
+  c.W = a.W;
+  c.od = 1;
+  c.in_os_hi = a.in_os;
+  c.out_os_hi = a.out_os;
+  c.in_os_lo = c.out_os_lo = 0;
+  c.in_is_lo = (unsigned)(a.in_is * (long long)sizeof(C2<TI>));
+  c.out_is_lo = (unsigned)(a.out_is * (long long)sizeof(C2<TO>));
+  c.in_is_hi = c.in_is_lo * (unsigned)pc.R1;
+  c.out_is_hi = c.out_is_lo * (unsigned)pc.R1;
+  c.gym = a.gym; c.gya = a.gya; c.zpar = a.zpar; c.Y2 = a.Y2; c.GZ1 = a.GZ1;
+  c.ncb = e->gc_ncb; c.npar = e->gc_npar; c.par = e->gc_npar == 2 ? a.gxa : 0;
+  c.pair = (MODE == PM_CONV && e->gc_npar == 1) ? 1 : 0;
+  L.T2L = sizeof(TC) == 8 ? (const void*)e->dtw[axis] : (const void*)e->dtw32[axis];
+  L.G = gc_of<TC>(e);
+  constexpr int combo = combo_of<TI, TC, TO>();
+  e->kbegin();
+  hipError_t err = hipErrorInvalidValue;
+  switch (pc.R1 * 64 + pc.R2) {
+#define CLAT_C2_CASE(ra, rb) \
+  case ra * 64 + rb: err = clat_c2_##ra##x##rb(combo, MODE, a.nsub, L); break;
+    CLAT_P2_PAIRS(CLAT_C2_CASE)
+#undef CLAT_C2_CASE
+    default: break;
+  }
+  HIPX(err);
+  e->kend(kind, bytes);
+  return true;
+}
+
 template <typename TI, typename TC, typename TO, int MODE, bool LINE>
 void launch2(clat_engine* e, const PassCfg& pc, P2Args a, int n_outer, int axis, const void* in, void* out,
              int kind, double bytes) {
+  if constexpr (!LINE) {
+    if (launch_col<TI, TC, TO, MODE>(e, pc, a, n_outer, axis, in, out, kind, bytes)) return;
+  }
   a.R1 = pc.R1;
   a.R2 = pc.R2;
   a.R1p = pc.R1p;
@@ -465,17 +536,18 @@ void prepare_fast(clat_engine* e, const double2* sites_in, double2* sites_out) {
 
   if (e->shared()) {
     const long long W = 2LL * Xi;
+    // B holds the y-forward output as [fy][z][fx]: the z pass reads rows W apart
     xfwd(2, 0, 0, 0, e->A);
     {
       P2Args a = base_args(e);
       a.L = Yi; a.nsub = 2; a.W = (int)W;
-      a.in_os = (long long)Yi * W; a.in_is = W; a.out_os = 2LL * Yi * W; a.out_is = W;
+      a.in_os = (long long)Yi * W; a.in_is = W; a.out_os = W; a.out_is = (long long)Zi * W;
       launch2<TA, TC, TB, PM_FWD, false>(e, e->cyf, a, Zi, 1, e->A, e->B, K_YFWD, (sa * 2 + sb * 4) * N);
     }
     {
       P2Args a = base_args(e);
       a.L = Zi; a.nsub = 2; a.W = (int)W;
-      a.in_os = a.out_os = W; a.in_is = a.out_is = 2LL * Yi * W;
+      a.in_os = a.out_os = (long long)Zi * W; a.in_is = a.out_is = W;
       a.gxm = 1; a.gym = 1;
       launch2<TB, TCZ, TB, PM_CONV, false>(e, e->czc, a, 2 * Yi, 2, e->B, e->B, K_ZCONV,
                                            sb * 8 * N + (double)sizeof(TCZ) * (double)e->NG());
@@ -483,7 +555,7 @@ void prepare_fast(clat_engine* e, const double2* sites_in, double2* sites_out) {
     {
       P2Args a = base_args(e);
       a.L = Yi; a.nsub = 2; a.W = (int)W;
-      a.in_os = 2LL * Yi * W; a.in_is = W; a.out_os = (long long)Yi * W; a.out_is = W;
+      a.in_os = W; a.in_is = (long long)Zi * W; a.out_os = (long long)Yi * W; a.out_is = W;
       launch2<TB, TCY, TD, PM_INV, false>(e, e->cyi, a, Zi, 1, e->B, e->D, K_YINV, (sb * 4 + sd * 2) * N);
     }
     xinv(2, 0, 0, 0, 0, e->D);
@@ -746,12 +818,19 @@ void finish_interaction(clat_engine* e) {
     hipLaunchKernelGGL(k_d2f, dim3(grid1d(e->NG())), dim3(256), 0, e->stream, e->G64, e->G32, e->NG());
     LAUNCH_CHECK();
   }
-  if (e->Gc64) {
+  if (e->conv_tiled) {
+    if (e->Gc64)
+      hipLaunchKernelGGL(k_gconv_tiled<double>, dim3(grid1d(e->gc_elems)), dim3(256), 0, e->stream, e->G64,
+                         e->Gc64, e->Xi, e->Yi + 1, e->Zi + 1, e->gc_c, e->gc_ncb, e->gc_npar, e->gc_w);
+    else
+      hipLaunchKernelGGL(k_gconv_tiled<float>, dim3(grid1d(e->gc_elems)), dim3(256), 0, e->stream, e->G64,
+                         e->Gc32, e->Xi, e->Yi + 1, e->Zi + 1, e->gc_c, e->gc_ncb, e->gc_npar, e->gc_w);
+    LAUNCH_CHECK();
+  } else if (e->Gc64) {
     hipLaunchKernelGGL(k_gconv<double>, dim3(grid1d(e->gc_elems)), dim3(256), 0, e->stream, e->G64, e->Gc64,
                        e->Xi, e->Yi + 1, e->Zi + 1);
     LAUNCH_CHECK();
-  }
-  if (e->Gc32) {
+  } else if (e->Gc32) {
     hipLaunchKernelGGL(k_gconv<float>, dim3(grid1d(e->gc_elems)), dim3(256), 0, e->stream, e->G64, e->Gc32,
                        e->Xi, e->Yi + 1, e->Zi + 1);
     LAUNCH_CHECK();
@@ -958,6 +1037,7 @@ int clat_create(int X, int Y, int Z, int precision, int schedule, clat_engine** 
               e->Xi <= 400 && e->Yi <= 400 && e->Zi <= 400;
     if (const char* g = std::getenv("CLAT_GENERIC"))
       if (std::atoi(g)) e->fast = false;
+    if (const char* g = std::getenv("CLAT_COLK")) e->colk = std::atoi(g) != 0;
     // the mixed mode needs the two-level kernels; otherwise it runs in FP64
     e->rprec = (precision == CLAT_PREC_FP32X && !e->fast) ? CLAT_PREC_FP64 : precision;
     if (const char* lb = std::getenv("CLAT_LDS_BYTES")) {
@@ -1020,6 +1100,9 @@ int clat_create(int X, int Y, int Z, int precision, int schedule, clat_engine** 
       e->tw64[d] = e->dalloc(tw.size() * sizeof(double));
       e->h64[d] = e->dalloc(hf.size() * sizeof(double));
       e->dtw[d] = (double2*)e->dalloc(dt.size() * sizeof(double));
+      std::vector<float> dt32(dt.begin(), dt.end());
+      e->dtw32[d] = (float2*)e->dalloc(dt32.size() * sizeof(float));
+      HIPX(hipMemcpy(e->dtw32[d], dt32.data(), dt32.size() * sizeof(float), hipMemcpyHostToDevice));
       HIPX(hipMemcpy(e->tw32[d], a.data(), a.size() * sizeof(float), hipMemcpyHostToDevice));
       HIPX(hipMemcpy(e->h32[d], b.data(), b.size() * sizeof(float), hipMemcpyHostToDevice));
       HIPX(hipMemcpy(e->tw64[d], tw.data(), tw.size() * sizeof(double), hipMemcpyHostToDevice));
@@ -1029,7 +1112,20 @@ int clat_create(int X, int Y, int Z, int precision, int schedule, clat_engine** 
     e->G64 = (double*)e->dalloc(e->NG() * sizeof(double));
     if (e->rprec == CLAT_PREC_FP32) e->G32 = (float*)e->dalloc(e->NG() * sizeof(float));
     if (e->fast) {
-      e->gc_elems = (size_t)2 * e->Xi * (e->Yi + 1) * (e->Zi + 1);
+      // conv pass on the column kernel when its radix pair is compiled and its rows fit 32-bit offsets
+      const long long zrow = e->shared() ? 2LL * e->Xi : (long long)e->Xi * e->Yi;
+      const size_t zb = e->shared() ? e->s_b : e->s_a;
+      e->conv_tiled = e->colk && col_pair_compiled(e->czc.R1, e->czc.R2) &&
+                      col_fits(e->Zi, nsub, zrow, zb, zrow, zb);
+      if (e->conv_tiled) {
+        e->gc_c = tcz == flt ? col_c<float>() : col_c<double>();
+        e->gc_npar = e->shared() ? 1 : 2;
+        e->gc_w = e->shared() ? 2 * e->Xi : e->Xi;
+        e->gc_ncb = (e->gc_w + e->gc_c - 1) / e->gc_c;
+        e->gc_elems = (size_t)(e->Yi + 1) * e->gc_npar * e->gc_ncb * (e->Zi + 1) * e->gc_c;
+      } else {
+        e->gc_elems = (size_t)2 * e->Xi * (e->Yi + 1) * (e->Zi + 1);
+      }
       if (tcz == flt) e->Gc32 = (float*)e->dalloc(e->gc_elems * sizeof(float));
       else e->Gc64 = (double*)e->dalloc(e->gc_elems * sizeof(double));
     }
@@ -1062,7 +1158,7 @@ int clat_destroy(clat_engine* e) {
   if (!e) return CLAT_OK;
   if (e->stream) (void)hipStreamSynchronize(e->stream);
   void* ptrs[] = {e->tw32[0], e->tw32[1], e->tw32[2], e->h32[0], e->h32[1], e->h32[2], e->tw64[0], e->tw64[1],
-                  e->tw64[2], e->h64[0], e->h64[1], e->h64[2], e->dtw[0], e->dtw[1], e->dtw[2], e->G64,
+                  e->tw64[2], e->h64[0], e->h64[1], e->h64[2], e->dtw[0], e->dtw[1], e->dtw[2], e->dtw32[0], e->dtw32[1], e->dtw32[2], e->G64,
                   e->G32, e->Gc64, e->Gc32, e->F, e->V, e->A, e->B, (e->D != e->A) ? e->D : nullptr, e->Fin, e->dpart,
                   e->d_site_atom, e->d_site_cell, e->d_site_ptr, e->d_atom_site, e->coeffs, e->rho, e->Vat,
                   e->part, e->ovl};

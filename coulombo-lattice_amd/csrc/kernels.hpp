@@ -347,7 +347,8 @@ __global__ void k_zero(C2<T>* __restrict__ p, size_t n) {
     p[t] = mk<T>((T)0, (T)0);
 }
 
-// Ĝ for the two-level conv pass: out[(ry*(Z1) + rz)*2X + fx] = Ghat[rz][ry][r(fx)]
+// Ĝ for the two-level conv pass in the old row layout:
+// out[(ry*Z1 + rz)*2X + fx] = Ghat[rz][ry][r(fx)]
 template <typename T>
 __global__ void k_gconv(const double* __restrict__ G, T* __restrict__ out, int X, int Y1, int Z1) {
   const size_t n = (size_t)2 * X * Y1 * Z1;
@@ -358,6 +359,33 @@ __global__ void k_gconv(const double* __restrict__ G, T* __restrict__ out, int X
     const int ry = (int)(q / Z1);
     const int rx = min(fx, 2 * X - fx);
     out[t] = (T)G[((size_t)rz * Y1 + ry) * (X + 1) + rx];
+  }
+}
+
+// Ĝ tiled for the compile-time column kernel: [ry][p][blk][rz][c] with
+// column col = blk*C + c of parity p, fx = col (shared schedule, npar = 1)
+// or 2 col + p (rounds schedule, npar = 2); columns past W hold zero.
+template <typename T>
+__global__ void k_gconv_tiled(const double* __restrict__ G, T* __restrict__ out, int X, int Y1, int Z1, int C,
+                              int ncb, int npar, int W) {
+  const size_t n = (size_t)Y1 * npar * ncb * Z1 * C;
+  for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < n; t += (size_t)gridDim.x * blockDim.x) {
+    const int c = (int)(t % C);
+    size_t q = t / C;
+    const int rz = (int)(q % Z1);
+    q /= Z1;
+    const int blk = (int)(q % ncb);
+    q /= ncb;
+    const int p = (int)(q % npar);
+    const int ry = (int)(q / npar);
+    const int col = blk * C + c;
+    T v = (T)0;
+    if (col < W) {
+      const int fx = npar == 1 ? col : 2 * col + p;
+      const int rx = min(fx, 2 * X - fx);
+      v = (T)G[((size_t)rz * Y1 + ry) * (X + 1) + rx];
+    }
+    out[t] = v;
   }
 }
 

@@ -69,6 +69,27 @@ def test_prepare_matches_oracle(prec, sched, dims):
     assert abs(r - ro) / abs(ro) < TOL[prec], (r, ro)
 
 
+@pytest.mark.parametrize("prec,sched", COMBOS, ids=IDS)
+@pytest.mark.parametrize("dims", [(3, 32, 64), (2, 45, 96), (3, 120, 160), (2, 192, 256), (4, 256, 128),
+                                  (2, 400, 8), (17, 64, 45)])
+def test_column_kernels_match_oracle(prec, sched, dims):
+    """y and z lengths on every compile-time radix pair of the column passes
+    (32, 45, 64, 96, 120, 128, 160, 192, 256, 400), with fewer columns than
+    one block holds (X = 2, 3) and a ragged last block (X = 17)."""
+    X, Y, Z = dims
+    print("case column", dims, prec, sched, flush=True)
+    rng = np.random.default_rng(X * 1000003 + Y * 1009 + Z)
+    F = rng.standard_normal((Z, Y, X)) + 1j * rng.standard_normal((Z, Y, X))
+    step = (0.8, 1.2, 1.0)
+    Vo = O.prepare(F, O.ghat(O.gtable(X, Y, Z, step, 0.5, 2.0)))
+    cc = clat.CoulombCalculator(dims, prec, sched)
+    cc.initialize(clat.InteractionSimple(step, 0.5, 2.0))
+    cc.input[...] = F
+    cc.prepare()
+    err = np.max(np.abs(cc.potential() - Vo)) / np.max(np.abs(Vo))
+    assert err < TOL[prec] * 0.1, err
+
+
 def _example():
     d = os.path.join(os.path.dirname(__file__), "golden", "example")
     xyz = np.loadtxt(os.path.join(d, "X.3d"), usecols=(0, 1, 2))
