@@ -31,6 +31,11 @@ struct ColArgs2 {
   // columns as [ry][parity p][column block][rz][C]; pair = 1 orders the
   // blocks so that fy and 2Y - fy (same table tile) run next to each other
   int gym, gya, zpar, Y2, GZ1, ncb, npar, par, pair;
+  // y passes of the atom path: word rowmask[o * mask_os + n1], bit n2 clear =
+  // n-domain row n1 + R1 n2 is an x-line without sites, never read (forward)
+  // or written (inverse)
+  const uint32_t* rowmask;
+  int mask_os;
 };
 
 template <typename T>
@@ -113,10 +118,11 @@ __global__ __launch_bounds__((col_threads<C, R1>())) void k_colpass(const C2<TI>
   if constexpr (MODE != PM_INV) {
     const bool ok = okc && r < R1;
     const uint32_t rbase = (uint32_t)r * a.in_is_lo + cin;
+    const uint32_t live = (a.rowmask && ok) ? a.rowmask[o * a.mask_os + r] : ~0u;
 #pragma unroll
     for (int n2 = 0; n2 < R2; ++n2) {
       const uint32_t off = rbase + (uint32_t)n2 * a.in_is_hi;
-      x[n2] = ok ? cvt<TI, TC>(ldg_off(bin, off)) : zero;
+      x[n2] = (ok && ((live >> n2) & 1u)) ? cvt<TI, TC>(ldg_off(bin, off)) : zero;
     }
   }
   // INV: thread (c, k2 = r) loads X_s[R2 k1 + k2] of every sub-line up front
@@ -235,10 +241,11 @@ __global__ __launch_bounds__((col_threads<C, R1>())) void k_colpass(const C2<TI>
   if constexpr (MODE != PM_FWD) {
     if (okc && r < R1) {
       const uint32_t rbase = (uint32_t)r * a.out_is_lo + cout;
+      const uint32_t live = a.rowmask ? a.rowmask[o * a.mask_os + r] : ~0u;
 #pragma unroll
       for (int n2 = 0; n2 < R2; ++n2) {
         const uint32_t off = rbase + (uint32_t)n2 * a.out_is_hi;
-        stg_off(bout, off, cvt<TC, TO>(acc[n2]));
+        if ((live >> n2) & 1u) stg_off(bout, off, cvt<TC, TO>(acc[n2]));
       }
     }
   }

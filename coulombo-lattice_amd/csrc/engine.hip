@@ -13,7 +13,7 @@
 #include "../../include/clat.h"
 #include "host_logic.hpp"
 #include "kernels.hpp"
-#include "colpass.hpp"
+#include "linepass.hpp"
 
 using namespace clat;
 
@@ -252,6 +252,14 @@ struct clat_engine {
   int32_t* d_site_cell = nullptr;    // engine-grid cell of each site
   int32_t* d_site_ptr = nullptr;     // CSR of sites over engine x-lines (Yi*Zi + 1)
   int32_t* d_atom_site = nullptr;    // site holding each atom's cell
+  // occupied x-lines (atom path of the shared schedule)
+  int64_t n_live = 0;
+  int32_t* d_lines = nullptr;        // line id of each occupied line, ascending
+  int32_t* d_live_ptr = nullptr;     // CSR of the sites over occupied lines (n_live + 1)
+  int32_t* d_site_ccell = nullptr;   // occupied-line index * Xi + x of each site
+  uint32_t* d_linemask = nullptr;    // y passes: word (z, n1), bit n2 = line y = n1 + R1y n2 holds a site
+  size_t linemask_words = 0;
+  bool xline = false;                // x passes on the line kernel, y passes skip empty lines
   int n_states = 0, cap_states = 0;
   double2* coeffs = nullptr;
   double2* rho = nullptr;
@@ -407,6 +415,8 @@ bool launch_col(clat_engine* e, const PassCfg& pc, const P2Args& a, int n_outer,
   c.gym = a.gym; c.gya = a.gya; c.zpar = a.zpar; c.Y2 = a.Y2; c.GZ1 = a.GZ1;
   c.ncb = e->gc_ncb; c.npar = e->gc_npar; c.par = e->gc_npar == 2 ? a.gxa : 0;
   c.pair = (MODE == PM_CONV && e->gc_npar == 1) ? 1 : 0;
+  c.rowmask = a.rowmask;
+  c.mask_os = a.mask_os;
   L.T2L = sizeof(TC) == 8 ? (const void*)e->dtw[axis] : (const void*)e->dtw32[axis];
   L.G = gc_of<TC>(e);
   constexpr int combo = combo_of<TI, TC, TO>();
@@ -430,6 +440,7 @@ void launch2(clat_engine* e, const PassCfg& pc, P2Args a, int n_outer, int axis,
   if constexpr (!LINE) {
     if (launch_col<TI, TC, TO, MODE>(e, pc, a, n_outer, axis, in, out, kind, bytes)) return;
   }
+  if (a.rowmask) throw host::Error(CLAT_EDEVICE, "row mask on the general pass kernel");
   a.R1 = pc.R1;
   a.R2 = pc.R2;
   a.R1p = pc.R1p;
@@ -465,6 +476,47 @@ void launch2(clat_engine* e, const PassCfg& pc, P2Args a, int n_outer, int axis,
   e->kend(kind, bytes);
 }
 
+// x pass of the atom path on the line kernel (occupied lines only)
+template <typename TI, typename TC, typename TO, int MODE>
+void launch_line(clat_engine* e, const void* in, void* out, const double2* sites_in, double2* sites_out, int kind,
+                 double bytes) {
+  LineLaunch L;
+  L.stream = e->stream;
+  L.in = in;
+  L.out = out;
+  LineArgs& a = L.a;
+  a.n_live = (int)e->n_live;
+  a.lines = e->d_lines;
+  a.live_ptr = e->d_live_ptr;
+  a.site_ccell = e->d_site_ccell;
+  a.site_in = sites_in;
+  a.site_out = sites_out;
+  a.in_ls = a.out_ls = 2LL * e->Xi;
+  L.T2L = sizeof(TC) == 8 ? (const void*)e->dtw[0] : (const void*)e->dtw32[0];
+  constexpr int combo = combo_of<TI, TC, TO>();
+  const int R1 = e->cxf_s.R1, R2 = e->cxf_s.R2;
+  int lines = 0;
+  switch (R1 * 64 + R2) {
+#define CLAT_L2_N(ra, rb) case ra * 64 + rb: lines = clat_l2_lines_##ra##x##rb(sizeof(TC) == 8); break;
+    CLAT_P2_PAIRS(CLAT_L2_N)
+#undef CLAT_L2_N
+    default: break;
+  }
+  if (lines <= 0) throw host::Error(CLAT_EDEVICE, "line kernel missing for the x radix pair");
+  if (e->n_live == 0) return;
+  L.blocks = (unsigned)((e->n_live + lines - 1) / lines);
+  e->kbegin();
+  hipError_t err = hipErrorInvalidValue;
+  switch (R1 * 64 + R2) {
+#define CLAT_L2_CASE(ra, rb) case ra * 64 + rb: err = clat_l2_##ra##x##rb(combo, MODE, L); break;
+    CLAT_P2_PAIRS(CLAT_L2_CASE)
+#undef CLAT_L2_CASE
+    default: break;
+  }
+  HIPX(err);
+  e->kend(kind, bytes);
+}
+
 P2Args base_args(const clat_engine* e) {
   P2Args a;
   std::memset(&a, 0, sizeof a);
@@ -492,6 +544,10 @@ void prepare_fast(clat_engine* e, const double2* sites_in, double2* sites_out) {
   const double site_bytes = (double)e->n_sites * 20.0;  // value + cell index
   const int nlines = Yi * Zi;
   const bool atoms = sites_in != nullptr;
+  // atom path of the shared schedule: x passes over occupied lines only, y
+  // passes never touch the rows of empty lines
+  const bool lk = atoms && e->shared() && e->xline;
+  const double live = lk ? (double)e->n_live / (double)nlines : 1.0;
 
   auto xfwd = [&](int nsub, int kx, int ky, int kz, void* out) {
     P2Args a = base_args(e);
@@ -502,8 +558,10 @@ void prepare_fast(clat_engine* e, const double2* sites_in, double2* sites_out) {
     a.out_os = (long long)nsub * Xi; a.out_is = 1;
     a.rphase = nsub == 1 ? 1 : 0;
     a.kx = kx; a.ky = ky; a.kz = kz;
-    const double bytes = (atoms ? site_bytes : si * N) + sa * N * nsub;
-    if (atoms) {
+    const double bytes = (atoms ? site_bytes : si * N) + sa * N * nsub * live;
+    if (lk) {
+      launch_line<double, TC, TA, PM_FWD>(e, nullptr, out, sites_in, nullptr, K_XFWD, bytes);
+    } else if (atoms) {
       a.site_ptr = e->d_site_ptr;
       a.site_cell = e->d_site_cell;
       a.site_in = sites_in;
@@ -522,7 +580,10 @@ void prepare_fast(clat_engine* e, const double2* sites_in, double2* sites_out) {
     a.rphase = nsub == 1 ? 1 : 0;
     a.kx = kx; a.ky = ky; a.kz = kz;
     a.acc = acc;
-    if (atoms) {
+    if (lk) {
+      launch_line<TD, TCX, double, PM_INV>(e, in, nullptr, nullptr, sites_out, K_XINV,
+                                           sd * N * nsub * live + site_bytes);
+    } else if (atoms) {
       a.site_ptr = e->d_site_ptr;
       a.site_cell = e->d_site_cell;
       a.site_out = sites_out;
@@ -542,7 +603,8 @@ void prepare_fast(clat_engine* e, const double2* sites_in, double2* sites_out) {
       P2Args a = base_args(e);
       a.L = Yi; a.nsub = 2; a.W = (int)W;
       a.in_os = (long long)Yi * W; a.in_is = W; a.out_os = W; a.out_is = (long long)Zi * W;
-      launch2<TA, TC, TB, PM_FWD, false>(e, e->cyf, a, Zi, 1, e->A, e->B, K_YFWD, (sa * 2 + sb * 4) * N);
+      if (lk) { a.rowmask = e->d_linemask; a.mask_os = e->cyf.R1; }
+      launch2<TA, TC, TB, PM_FWD, false>(e, e->cyf, a, Zi, 1, e->A, e->B, K_YFWD, (sa * 2 * live + sb * 4) * N);
     }
     {
       P2Args a = base_args(e);
@@ -556,7 +618,8 @@ void prepare_fast(clat_engine* e, const double2* sites_in, double2* sites_out) {
       P2Args a = base_args(e);
       a.L = Yi; a.nsub = 2; a.W = (int)W;
       a.in_os = W; a.in_is = (long long)Zi * W; a.out_os = (long long)Yi * W; a.out_is = W;
-      launch2<TB, TCY, TD, PM_INV, false>(e, e->cyi, a, Zi, 1, e->B, e->D, K_YINV, (sb * 4 + sd * 2) * N);
+      if (lk) { a.rowmask = e->d_linemask; a.mask_os = e->cyi.R1; }
+      launch2<TB, TCY, TD, PM_INV, false>(e, e->cyi, a, Zi, 1, e->B, e->D, K_YINV, (sb * 4 + sd * 2 * live) * N);
     }
     xinv(2, 0, 0, 0, 0, e->D);
   } else {
@@ -919,6 +982,12 @@ void free_atoms(clat_engine* e) {
   e->dfree(e->d_site_cell, (size_t)e->n_sites * 4);
   e->dfree(e->d_site_ptr, ((size_t)e->Yi * e->Zi + 1) * 4);
   e->dfree(e->d_atom_site, (size_t)e->n_atoms * 4);
+  e->dfree(e->d_lines, std::max<size_t>(1, (size_t)e->n_live) * 4);
+  e->dfree(e->d_live_ptr, ((size_t)e->n_live + 1) * 4);
+  e->dfree(e->d_site_ccell, std::max<size_t>(1, (size_t)e->n_sites) * 4);
+  e->dfree(e->d_linemask, e->linemask_words * 4);
+  e->linemask_words = 0;
+  e->n_live = 0;
   e->dfree(e->coeffs, (size_t)e->cap_states * e->n_atoms * e->n_orb * 16);
   e->dfree(e->rho, e->rho_bytes);
   e->rho_bytes = 0;
@@ -1089,6 +1158,14 @@ int clat_create(int X, int Y, int Z, int precision, int schedule, clat_engine** 
       if (e->cxf_s.C * e->cxf_s.R1 > e->cxf_s.NT || e->cxi_s.C * e->cxi_s.R1 > e->cxi_s.NT)
         throw host::Error(CLAT_EUNSUP, "site pass shape");
     }
+    if (e->fast && e->shared() && e->colk) {
+      const long long W = 2LL * e->Xi;
+      e->xline = col_pair_compiled(e->cxf_s.R1, e->cxf_s.R2) && col_pair_compiled(e->cyf.R1, e->cyf.R2) &&
+                 e->cyf.R1 == e->cyi.R1 && e->cyf.R2 == e->cyi.R2 &&
+                 col_fits(e->Yi, 2, W, e->s_a, (long long)e->Zi * W, e->s_b) &&
+                 col_fits(e->Yi, 2, (long long)e->Zi * W, e->s_b, W, e->s_d);
+    }
+    if (const char* g = std::getenv("CLAT_XLINE")) e->xline = e->xline && std::atoi(g) != 0;
     HIPX(hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking));
     for (int i = 0; i < 6; ++i) HIPX(hipEventCreate(&e->stage_ev[i]));
     for (int d = 0; d < 3; ++d) {
@@ -1160,7 +1237,8 @@ int clat_destroy(clat_engine* e) {
   void* ptrs[] = {e->tw32[0], e->tw32[1], e->tw32[2], e->h32[0], e->h32[1], e->h32[2], e->tw64[0], e->tw64[1],
                   e->tw64[2], e->h64[0], e->h64[1], e->h64[2], e->dtw[0], e->dtw[1], e->dtw[2], e->dtw32[0], e->dtw32[1], e->dtw32[2], e->G64,
                   e->G32, e->Gc64, e->Gc32, e->F, e->V, e->A, e->B, (e->D != e->A) ? e->D : nullptr, e->Fin, e->dpart,
-                  e->d_site_atom, e->d_site_cell, e->d_site_ptr, e->d_atom_site, e->coeffs, e->rho, e->Vat,
+                  e->d_site_atom, e->d_site_cell, e->d_site_ptr, e->d_atom_site, e->d_lines, e->d_live_ptr,
+                  e->d_site_ccell, e->d_linemask, e->coeffs, e->rho, e->Vat,
                   e->part, e->ovl};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
@@ -1330,6 +1408,25 @@ int clat_set_atoms(clat_engine* e, const int32_t* cell, int64_t n_atoms, int n_o
     }
     for (int64_t l = 0; l < nlines; ++l) site_ptr[(size_t)l + 1] += site_ptr[(size_t)l];
     for (int64_t a = 0; a < n_atoms; ++a) atom_site[(size_t)a] = last[(size_t)cell[a]];
+    // occupied x-lines: compact list, CSR over it, compact cells, bit mask
+    std::vector<int32_t> lines, live_ptr(1, 0), ccell(site_cell.size());
+    // y-pass mask: one word per (z, n1) of the y radix split, bit n2 for y = n1 + R1 n2
+    const int yr1 = e->cyf.R1, yr2 = e->cyf.R2;
+    const bool ymask = e->xline && yr2 <= 32;
+    std::vector<uint32_t> mask(ymask ? (size_t)e->Zi * yr1 : 1, 0u);
+    for (int64_t l = 0; l < nlines; ++l) {
+      const int32_t b = site_ptr[(size_t)l], n = site_ptr[(size_t)l + 1] - b;
+      if (n == 0) continue;
+      const int64_t rank = (int64_t)lines.size();
+      for (int32_t k = b; k < b + n; ++k)
+        ccell[(size_t)k] = (int32_t)(rank * e->Xi + (site_cell[(size_t)k] - l * e->Xi));
+      lines.push_back((int32_t)l);
+      live_ptr.push_back(b + n);
+      if (ymask) {
+        const int64_t z = l / e->Yi, y = l % e->Yi;
+        mask[(size_t)(z * yr1 + y % yr1)] |= 1u << (y / yr1);
+      }
+    }
     free_atoms(e);
     e->n_atoms = n_atoms;
     e->n_orb = n_orbitals;
@@ -1342,6 +1439,16 @@ int clat_set_atoms(clat_engine* e, const int32_t* cell, int64_t n_atoms, int n_o
     HIPX(hipMemcpy(e->d_site_cell, site_cell.data(), (size_t)e->n_sites * 4, hipMemcpyHostToDevice));
     HIPX(hipMemcpy(e->d_site_ptr, site_ptr.data(), ((size_t)nlines + 1) * 4, hipMemcpyHostToDevice));
     HIPX(hipMemcpy(e->d_atom_site, atom_site.data(), (size_t)n_atoms * 4, hipMemcpyHostToDevice));
+    e->n_live = (int64_t)lines.size();
+    e->d_lines = (int32_t*)e->dalloc(std::max<size_t>(1, lines.size()) * 4);
+    e->d_live_ptr = (int32_t*)e->dalloc(live_ptr.size() * 4);
+    e->d_site_ccell = (int32_t*)e->dalloc(std::max<size_t>(1, ccell.size()) * 4);
+    e->d_linemask = (uint32_t*)e->dalloc(mask.size() * 4);
+    e->linemask_words = mask.size();
+    if (!lines.empty()) HIPX(hipMemcpy(e->d_lines, lines.data(), lines.size() * 4, hipMemcpyHostToDevice));
+    HIPX(hipMemcpy(e->d_live_ptr, live_ptr.data(), live_ptr.size() * 4, hipMemcpyHostToDevice));
+    if (!ccell.empty()) HIPX(hipMemcpy(e->d_site_ccell, ccell.data(), ccell.size() * 4, hipMemcpyHostToDevice));
+    HIPX(hipMemcpy(e->d_linemask, mask.data(), mask.size() * 4, hipMemcpyHostToDevice));
   });
 }
 

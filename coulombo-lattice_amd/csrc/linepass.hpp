@@ -1,0 +1,220 @@
+// x passes of the atom path (shared schedule) with every size fixed at
+// compile time: radix pair R1 x R2, both half-shift parities (NSUB = 2) and C
+// lines per block.  Only x-lines that hold at least one atom site are visited:
+// a block takes C consecutive entries of the compact list of occupied lines.
+//
+//   forward: sites -> LDS line buffer -> two-level DFT of both parities ->
+//            A[line][(R2 k1 + k2) NSUB + s]
+//   inverse: D[line][...] -> two-level inverse DFT, parities combined ->
+//            LDS line buffer -> potential at the sites
+//
+// Index map (L = R1 R2): n = n1 + R1 n2, k = R2 k1 + k2; the half-shift phase
+// h[n] = W_2L^n is split into W_2L^{n1 (2 k2 + s)} (table T2L of 2L roots)
+// and a compile-time rotation by W_2R2^n2, as in the column passes.
+#pragma once
+
+#include "colpass.hpp"
+
+namespace clat {
+
+struct LineArgs {
+  int n_live;                   // occupied lines
+  const int32_t* lines;         // line id of each compact index
+  const int32_t* live_ptr;      // CSR of the sites over compact lines (n_live + 1)
+  const int32_t* site_ccell;    // compact cell of each site: compact line * L + x
+  const double2* site_in;       // forward: pair density at the sites
+  double2* site_out;            // inverse: potential at the sites
+  long long in_ls, out_ls;      // element stride between lines of the dense side
+};
+
+template <int C, int R1, int R2, int NSUB>
+constexpr int line_threads() {
+  constexpr int m = C * NSUB * R2 > C * R1 ? C * NSUB * R2 : C * R1;
+  return (m + 63) / 64 * 64;
+}
+// lines per block: about 256 threads in the phase with NSUB R2 items per
+// line, half as many lines when the arithmetic is FP64
+template <typename TC, int R2, int NSUB>
+constexpr int line_c() {
+  constexpr int q = (256 / (NSUB * R2)) / (sizeof(TC) == 8 ? 2 : 1);
+  return q >= 16 ? 16 : q >= 8 ? 8 : q >= 4 ? 4 : q >= 2 ? 2 : 1;
+}
+template <int R1>
+constexpr int line_r1p() {
+  return (R1 > 1 && R1 % 2 == 0) ? R1 + 1 : R1;
+}
+template <typename TC, int R1, int R2, int NSUB, int C>
+constexpr size_t line_lds() {
+  return (size_t)C * NSUB * R2 * line_r1p<R1>() * sizeof(C2<TC>);
+}
+
+template <typename TI, typename TC, typename TO, int MODE, int R1, int R2, int NSUB, int C>
+__global__ __launch_bounds__((line_threads<C, R1, R2, NSUB>())) void k_linepass(const C2<TI>* in, C2<TO>* out,
+                                                                                LineArgs a,
+                                                                                const C2<TC>* __restrict__ T2L) {
+  using CT = C2<TC>;
+  constexpr int L = R1 * R2, R1P = line_r1p<R1>();
+  constexpr int NTH = line_threads<C, R1, R2, NSUB>();
+  constexpr uint32_t SC = sizeof(CT);
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  CT* lds = reinterpret_cast<CT*>(smem_raw);
+  const int tid = threadIdx.x;
+  const int b0 = blockIdx.x * C;
+  const int nC = min(C, a.n_live - b0);
+  const CT zero = mk<TC>((TC)0, (TC)0);
+  const int s0 = a.live_ptr[b0], s1 = a.live_ptr[b0 + nC];
+  const int cell0 = b0 * L;
+  auto xi = [](int c, int s, int k2, int n1) { return ((c * NSUB + s) * R2 + k2) * R1P + n1; };
+
+  // (c, n1) items of the phases holding R2 values
+  const int c13 = tid / R1, n1 = tid - c13 * R1;
+  const bool ok13 = tid < C * R1 && c13 < nC;
+  // (c, s, k2) items of the phase holding R1 values
+  const int s2 = tid & (NSUB - 1), k2 = (tid / NSUB) % R2, c2 = tid / (NSUB * R2);
+  const bool ok2 = tid < C * NSUB * R2 && c2 < nC;
+
+  if constexpr (MODE == PM_FWD) {
+    // ---------------------------------------------------------- scatter
+    for (int e = tid; e < C * L; e += NTH) lds[e] = zero;
+    __syncthreads();
+    for (int k = s0 + tid; k < s1; k += NTH) {
+      const double2 v = a.site_in[k];
+      lds[a.site_ccell[k] - cell0] = mk<TC>((TC)v.x, (TC)v.y);
+    }
+    __syncthreads();
+    CT x[R2];
+#pragma unroll
+    for (int n2 = 0; n2 < R2; ++n2) x[n2] = ok13 ? lds[c13 * L + n1 + R1 * n2] : zero;
+    __syncthreads();
+    // ---------------------------------------------------------- P1
+    if (ok13) {
+#pragma unroll
+      for (int s = 0; s < NSUB; ++s) {
+        CT v[R2];
+#pragma unroll
+        for (int n2 = 0; n2 < R2; ++n2) v[n2] = x[n2];
+        if (s == 1) rot_all<R2, false, TC>(v);
+        rdft<R2, false, TC>(v);
+        const uint32_t tstep = (uint32_t)(2 * n1) * SC, tbase = (uint32_t)(s * n1) * SC;
+#pragma unroll
+        for (int q = 0; q < R2; ++q) {
+          CT t = v[q];
+          if (q > 0 || s == 1) t = cmul<TC>(t, ldg_off(T2L, tbase + (uint32_t)q * tstep));
+          lds[xi(c13, s, q, n1)] = t;
+        }
+      }
+    }
+    __syncthreads();
+    // ---------------------------------------------------------- P2 -> A
+    if (ok2) {
+      CT v[R1];
+#pragma unroll
+      for (int m = 0; m < R1; ++m) v[m] = lds[xi(c2, s2, k2, m)];
+      rdft<R1, false, TC>(v);
+      C2<TO>* o = out + (size_t)a.lines[b0 + c2] * a.out_ls + k2 * NSUB + s2;
+#pragma unroll
+      for (int k1 = 0; k1 < R1; ++k1) o[k1 * R2 * NSUB] = cvt<TC, TO>(v[k1]);
+    }
+  } else {
+    // ---------------------------------------------------------- P2 <- D
+    if (ok2) {
+      const C2<TI>* ip = in + (size_t)a.lines[b0 + c2] * a.in_ls + k2 * NSUB + s2;
+      CT v[R1];
+#pragma unroll
+      for (int k1 = 0; k1 < R1; ++k1) v[k1] = cvt<TI, TC>(ip[k1 * R2 * NSUB]);
+      rdft<R1, true, TC>(v);
+      const uint32_t tstep = (uint32_t)(2 * k2 + s2) * SC;
+#pragma unroll
+      for (int m = 0; m < R1; ++m) {
+        CT t = v[m];
+        if (m > 0) t = cmulc<TC>(t, ldg_off(T2L, (uint32_t)m * tstep));
+        lds[xi(c2, s2, k2, m)] = t;
+      }
+    }
+    __syncthreads();
+    // ---------------------------------------------------------- P3
+    CT acc[R2];
+#pragma unroll
+    for (int n2 = 0; n2 < R2; ++n2) acc[n2] = zero;
+    if (ok13) {
+#pragma unroll
+      for (int s = 0; s < NSUB; ++s) {
+        CT v[R2];
+#pragma unroll
+        for (int q = 0; q < R2; ++q) v[q] = lds[xi(c13, s, q, n1)];
+        rdft<R2, true, TC>(v);
+        if (s == 1) rot_all<R2, true, TC>(v);
+#pragma unroll
+        for (int n2 = 0; n2 < R2; ++n2) acc[n2] = cadd<TC>(acc[n2], v[n2]);
+      }
+    }
+    __syncthreads();
+    if (ok13) {
+#pragma unroll
+      for (int n2 = 0; n2 < R2; ++n2) lds[c13 * L + n1 + R1 * n2] = acc[n2];
+    }
+    __syncthreads();
+    // ---------------------------------------------------------- gather
+    for (int k = s0 + tid; k < s1; k += NTH) {
+      const CT v = lds[a.site_ccell[k] - cell0];
+      a.site_out[k] = make_double2((double)v.x, (double)v.y);
+    }
+  }
+}
+
+struct LineLaunch {
+  unsigned blocks;
+  hipStream_t stream;
+  const void* in;
+  void* out;
+  LineArgs a;
+  const void* T2L;
+};
+
+template <typename TI, typename TC, typename TO, int MODE, int R1, int R2>
+hipError_t line_go(const LineLaunch& L) {
+  constexpr int NSUB = 2;
+  constexpr int C = line_c<TC, R2, NSUB>();
+  constexpr int nth = line_threads<C, R1, R2, NSUB>();
+  constexpr size_t lds = line_lds<TC, R1, R2, NSUB, C>();
+  static_assert(lds <= 65536, "line pass tile exceeds 64 KiB of LDS");
+  hipLaunchKernelGGL((k_linepass<TI, TC, TO, MODE, R1, R2, NSUB, C>), dim3(L.blocks), dim3(nth), lds, L.stream,
+                     (const C2<TI>*)L.in, (C2<TO>*)L.out, L.a, (const C2<TC>*)L.T2L);
+  return hipGetLastError();
+}
+
+// lines per block of a pair, for the host's grid size
+template <int R1, int R2>
+int line_lines(bool fp64_arith) {
+  return fp64_arith ? line_c<double, R2, 2>() : line_c<float, R2, 2>();
+}
+
+// (types, mode) pairs of the atom-path x passes; hipErrorInvalidValue otherwise
+template <int R1, int R2>
+hipError_t line_launch(int combo, int mode, const LineLaunch& L) {
+  using d = double;
+  using f = float;
+  if (mode == PM_FWD) {
+    switch (combo) {
+      case CB_DDD: return line_go<d, d, d, PM_FWD, R1, R2>(L);
+      case CB_DFF: return line_go<d, f, f, PM_FWD, R1, R2>(L);
+      case CB_DDF: return line_go<d, d, f, PM_FWD, R1, R2>(L);
+      default: break;
+    }
+  } else if (mode == PM_INV) {
+    switch (combo) {
+      case CB_DDD: return line_go<d, d, d, PM_INV, R1, R2>(L);
+      case CB_FFD: return line_go<f, f, d, PM_INV, R1, R2>(L);
+      case CB_FDD: return line_go<f, d, d, PM_INV, R1, R2>(L);
+      default: break;
+    }
+  }
+  return hipErrorInvalidValue;
+}
+
+#define CLAT_L2_DECL(a, b)                                                       \
+  hipError_t clat_l2_##a##x##b(int combo, int mode, const LineLaunch& L);        \
+  int clat_l2_lines_##a##x##b(bool fp64_arith);
+CLAT_P2_PAIRS(CLAT_L2_DECL)
+
+}  // namespace clat

@@ -41,6 +41,9 @@ struct P2Args {
   const int32_t* site_cell;
   const double2* site_in;   // forward: pair density at the sites (replaces `in`)
   double2* site_out;        // inverse: potential at the sites (replaces `out`)
+  // rows without sites (column kernel only; the general kernel rejects it)
+  const uint32_t* rowmask;
+  int mask_os;
 };
 
 // register DFT of size R: compile-time when RT > 0, else dispatched on R

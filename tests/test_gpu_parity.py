@@ -161,6 +161,38 @@ def test_potential_at_atoms(prec):
 
 
 @pytest.mark.parametrize("prec", [clat.PREC_FP64, clat.PREC_FP32, clat.PREC_FP32X], ids=["fp64", "fp32", "fp32x"])
+@pytest.mark.parametrize("dims", [(32, 6, 5), (45, 4, 3), (64, 3, 7), (96, 5, 2), (120, 3, 3), (128, 4, 4),
+                                  (160, 2, 3), (192, 3, 2), (256, 5, 3), (400, 2, 2)])
+def test_site_lines_match_oracle(prec, dims):
+    """Atom path on x lengths of every compile-time radix pair: random sites
+    on a random subset of x-lines (empty lines skipped by the x passes and
+    never touched by the y passes), several atoms per cell (the last one
+    holds the cell), an occupied-line count that leaves a ragged last block."""
+    X, Y, Z = dims
+    print("case site_lines", dims, prec, flush=True)
+    rng = np.random.default_rng(X * 7919 + Y * 31 + Z)
+    lines = rng.choice(Y * Z, size=max(1, (Y * Z * 3) // 5), replace=False)
+    cells = []
+    for l in lines:
+        xs = rng.choice(X, size=int(rng.integers(1, max(2, X // 3))), replace=False)
+        cells += [int(l) * X + int(x) for x in xs]
+    cells += cells[: len(cells) // 7]  # repeated cells: the later atom holds the cell
+    cells = np.array(cells, dtype=np.int32)
+    n_orb = 2
+    psi = rng.standard_normal((len(cells), n_orb)) + 1j * rng.standard_normal((len(cells), n_orb))
+    step = (1.0, 1.1, 0.9)
+    eng = clat.Engine((X, Y, Z), prec, clat.SCHED_SHARED)
+    eng.initialize(clat.InteractionSimple(step, 0.3, 2.5))
+    eng.set_atoms(cells, n_orb)
+    eng.add_state(psi)
+    v = eng.potential_at_atoms(1)
+    F = np.zeros(X * Y * Z, complex)
+    F[cells] = np.sum(np.abs(psi) ** 2, axis=1)
+    Vo = O.prepare(F.reshape(Z, Y, X), O.ghat(O.gtable(X, Y, Z, step, 0.3, 2.5))).reshape(-1)[cells]
+    assert np.max(np.abs(v - Vo)) / np.max(np.abs(Vo)) < TOL[prec]
+
+
+@pytest.mark.parametrize("prec", [clat.PREC_FP64, clat.PREC_FP32, clat.PREC_FP32X], ids=["fp64", "fp32", "fp32x"])
 def test_thomas_fermi(prec):
     """InteractionThomasFermi (Interaction.cpp:18-39) table vs the oracle's."""
     print("case thomas_fermi", prec, flush=True)
