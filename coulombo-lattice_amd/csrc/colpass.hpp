@@ -112,13 +112,23 @@ __global__ __launch_bounds__((col_threads<C, R1>())) void k_colpass(const C2<TI>
   C2<TO>* bout = out + (size_t)oh * a.out_os_hi + (size_t)ol * a.out_os_lo + (size_t)blockIdx.x * C;
   const uint32_t cin = (uint32_t)c * SI, cout = (uint32_t)c * SO;
 
+  // pass twiddles W_2L^m, m < 2L, go to LDS: read there by every phase, they
+  // cost no vector-memory instructions next to the column traffic
+  constexpr int NTH = col_threads<C, R1>();
+  constexpr int NTW = (2 * L + NTH - 1) / NTH;
+  __shared__ CT twl[2 * L];
+  CT twv[NTW];
+#pragma unroll
+  for (int j = 0; j < NTW; ++j)
+    if (tid + j * NTH < 2 * L) twv[j] = T2L[tid + j * NTH];
+
   // ------------------------------------------------------------------ load
   // FWD / CONV: thread (c, n1 = r) loads x[n1 + R1 n2], n2 < R2
   CT x[MODE == PM_INV ? 1 : R2];
   if constexpr (MODE != PM_INV) {
     const bool ok = okc && r < R1;
     const uint32_t rbase = (uint32_t)r * a.in_is_lo + cin;
-    const uint32_t live = (a.rowmask && ok) ? a.rowmask[o * a.mask_os + r] : ~0u;
+    const uint32_t live = (MODE == PM_FWD && a.rowmask && ok) ? a.rowmask[o * a.mask_os + r] : ~0u;
 #pragma unroll
     for (int n2 = 0; n2 < R2; ++n2) {
       const uint32_t off = rbase + (uint32_t)n2 * a.in_is_hi;
@@ -137,6 +147,14 @@ __global__ __launch_bounds__((col_threads<C, R1>())) void k_colpass(const C2<TI>
         xi[s * R1 + k1] = ok ? cvt<TI, TC>(ldg_off(bin, off)) : zero;
       }
   }
+
+#pragma unroll
+  for (int j = 0; j < NTW; ++j)
+    if (tid + j * NTH < 2 * L) twl[tid + j * NTH] = twv[j];
+  __syncthreads();
+  auto TW = [&](uint32_t byte_off) {
+    return *reinterpret_cast<const CT*>(reinterpret_cast<const char*>(twl) + byte_off);
+  };
 
   CT acc[MODE == PM_FWD ? 1 : R2];
   if constexpr (MODE != PM_FWD) {
@@ -167,7 +185,7 @@ __global__ __launch_bounds__((col_threads<C, R1>())) void k_colpass(const C2<TI>
 #pragma unroll
         for (int k2 = 0; k2 < R2; ++k2) {
           CT t = v[k2];
-          if (k2 > 0 || s == 1) t = cmul<TC>(t, ldg_off(T2L, tbase + (uint32_t)k2 * tstep));
+          if (k2 > 0 || s == 1) t = cmul<TC>(t, TW(tbase + (uint32_t)k2 * tstep));
           lds[(k2 * R1 + r) * C + c] = t;
         }
       }
@@ -216,7 +234,7 @@ __global__ __launch_bounds__((col_threads<C, R1>())) void k_colpass(const C2<TI>
 #pragma unroll
         for (int n1 = 0; n1 < R1; ++n1) {
           CT t = v[n1];
-          if (n1 > 0) t = cmulc<TC>(t, ldg_off(T2L, (uint32_t)n1 * tstep));
+          if (n1 > 0) t = cmulc<TC>(t, TW((uint32_t)n1 * tstep));
           lds[(r * R1 + n1) * C + c] = t;
         }
       }
@@ -241,7 +259,7 @@ __global__ __launch_bounds__((col_threads<C, R1>())) void k_colpass(const C2<TI>
   if constexpr (MODE != PM_FWD) {
     if (okc && r < R1) {
       const uint32_t rbase = (uint32_t)r * a.out_is_lo + cout;
-      const uint32_t live = a.rowmask ? a.rowmask[o * a.mask_os + r] : ~0u;
+      const uint32_t live = (MODE == PM_INV && a.rowmask) ? a.rowmask[o * a.mask_os + r] : ~0u;
 #pragma unroll
       for (int n2 = 0; n2 < R2; ++n2) {
         const uint32_t off = rbase + (uint32_t)n2 * a.out_is_hi;

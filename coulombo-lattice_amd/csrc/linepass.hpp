@@ -66,6 +66,7 @@ __global__ __launch_bounds__((line_threads<C, R1, R2, NSUB>())) void k_linepass(
   const int cell0 = b0 * L;
   auto xi = [](int c, int s, int k2, int n1) { return ((c * NSUB + s) * R2 + k2) * R1P + n1; };
 
+
   // (c, n1) items of the phases holding R2 values
   const int c13 = tid / R1, n1 = tid - c13 * R1;
   const bool ok13 = tid < C * R1 && c13 < nC;
@@ -74,6 +75,12 @@ __global__ __launch_bounds__((line_threads<C, R1, R2, NSUB>())) void k_linepass(
   const bool ok2 = tid < C * NSUB * R2 && c2 < nC;
 
   if constexpr (MODE == PM_FWD) {
+    // pass twiddles W_2L^m, m < 2L, in LDS (the scatter's barriers cover the fill)
+    __shared__ CT twl[2 * L];
+    for (int i = tid; i < 2 * L; i += NTH) twl[i] = T2L[i];
+    auto TW = [&](uint32_t byte_off) {
+      return *reinterpret_cast<const CT*>(reinterpret_cast<const char*>(twl) + byte_off);
+    };
     // ---------------------------------------------------------- scatter
     for (int e = tid; e < C * L; e += NTH) lds[e] = zero;
     __syncthreads();
@@ -99,7 +106,7 @@ __global__ __launch_bounds__((line_threads<C, R1, R2, NSUB>())) void k_linepass(
 #pragma unroll
         for (int q = 0; q < R2; ++q) {
           CT t = v[q];
-          if (q > 0 || s == 1) t = cmul<TC>(t, ldg_off(T2L, tbase + (uint32_t)q * tstep));
+          if (q > 0 || s == 1) t = cmul<TC>(t, TW(tbase + (uint32_t)q * tstep));
           lds[xi(c13, s, q, n1)] = t;
         }
       }
@@ -117,9 +124,11 @@ __global__ __launch_bounds__((line_threads<C, R1, R2, NSUB>())) void k_linepass(
     }
   } else {
     // ---------------------------------------------------------- P2 <- D
+    // (the twiddles come from global memory here: a barrier before this phase
+    // costs more than their loads)
+    CT v[R1];
     if (ok2) {
       const C2<TI>* ip = in + (size_t)a.lines[b0 + c2] * a.in_ls + k2 * NSUB + s2;
-      CT v[R1];
 #pragma unroll
       for (int k1 = 0; k1 < R1; ++k1) v[k1] = cvt<TI, TC>(ip[k1 * R2 * NSUB]);
       rdft<R1, true, TC>(v);

@@ -32,6 +32,12 @@ def kind_of(name):
         mode, line = int(m.group(4)), m.group(5) == "true"
         return {(0, True): "xfwd", (0, False): "yfwd", (1, False): "zconv",
                 (2, False): "yinv", (2, True): "xinv"}[(mode, line)]
+    m = re.search(r"k_colpass<(\w+), (\w+), (\w+), (\d),", name)
+    if m:
+        return {0: "yfwd", 1: "zconv", 2: "yinv"}[int(m.group(4))]
+    m = re.search(r"k_linepass<(\w+), (\w+), (\w+), (\d),", name)
+    if m:
+        return {0: "xfwd", 2: "xinv"}[int(m.group(4))]
     for key in ("k_products_direct", "k_products", "k_overlap_partial", "k_overlap_reduce", "k_dct1",
                 "k_gconv", "k_gtable", "k_gpad", "k_d2f", "k_xfwd", "k_xinv", "k_col", "k_grid_dot",
                 "k_atoms_from_sites", "copyBuffer"):
@@ -41,7 +47,7 @@ def kind_of(name):
 
 
 def short(name):
-    m = re.search(r"(k_pass2<[^>]*>)", name)
+    m = re.search(r"(k_(?:pass2|colpass|linepass)<[^>]*>)", name)
     return m.group(1) if m else name.split("(")[0].replace("void ", "")[:70]
 
 
@@ -75,6 +81,7 @@ def main():
     ap.add_argument("--write", help="output dir of the WRITE_SIZE run")
     ap.add_argument("--copy-fetch", help="FETCH_SIZE run of bench.py --copy-probe")
     ap.add_argument("--copy-write", help="WRITE_SIZE run of bench.py --copy-probe")
+    ap.add_argument("--rdreq", help="run with TCC_EA0_RDREQ_{32B,64B,128B}_sum (read requests by size)")
     ap.add_argument("--copy-mib", type=float, default=1024.0)
     ap.add_argument("--title", default="kernel trace")
     ap.add_argument("--commands", default="", help="text block naming the commands that made the inputs")
@@ -102,7 +109,8 @@ def main():
         fetch, write = counters(a.fetch, "FETCH_SIZE"), counters(a.write, "WRITE_SIZE")
         per_kind = {}
         for k in fetch:
-            kd = kind_of(k) if "k_pass2<" in k else short(k).replace("clat::", "")
+            kd = kind_of(k) if ("k_pass2<" in k or "k_colpass<" in k or "k_linepass<" in k) \
+                else short(k).replace("clat::", "")
             fk, wk = fetch[k], write.get(k, 0.0)
             per_kind[kd] = {"kernel": short(k), "fetch_kib": fk, "write_kib": wk,
                             "hbm_bytes_per_launch": fk * 1024.0 * f_factor + wk * 1024.0 * w_factor}
@@ -116,6 +124,22 @@ def main():
         for kd, v in sorted(per_kind.items()):
             out.append("| %s | `%s` | %.0f | %.0f | %.1f |" % (kd, v["kernel"], v["fetch_kib"], v["write_kib"],
                                                                v["hbm_bytes_per_launch"] / 1e6))
+        if a.rdreq:
+            # read bytes from the request-size counters: 32, 64 and 128-byte requests
+            r32, r64, r128 = (counters(a.rdreq, "TCC_EA0_RDREQ_%s_sum" % b) for b in ("32B", "64B", "128B"))
+            out += ["", "Cross-check: HBM read bytes from the request-size counters (32·RDREQ_32B + "
+                    "64·RDREQ_64B + 128·RDREQ_128B) against the scaled FETCH_SIZE.", "",
+                    "| kind | 128-B requests | 64-B | 32-B | read MB (requests) | read MB (FETCH_SIZE x factor) |",
+                    "|---|---:|---:|---:|---:|---:|"]
+            for k in r128:
+                kd = kind_of(k) if "k_pass2<" in k or "k_colpass<" in k or "k_linepass<" in k \
+                    else short(k).replace("clat::", "")
+                rb = 32 * r32.get(k, 0) + 64 * r64.get(k, 0) + 128 * r128[k]
+                fb = fetch.get(k, 0.0) * 1024.0 * f_factor
+                out.append("| %s | %.0f | %.0f | %.0f | %.1f | %.1f |" % (kd, r128[k], r64.get(k, 0), r32.get(k, 0),
+                                                                        rb / 1e6, fb / 1e6))
+                if kd in per_kind:
+                    per_kind[kd]["read_bytes_by_request_size"] = rb
     with open(a.md, "w") as f:
         f.write("\n".join(out) + "\n")
     if traffic is not None and a.traffic_json:
