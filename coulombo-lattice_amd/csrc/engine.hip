@@ -954,17 +954,18 @@ void build_products(clat_engine* e) {
   e->rho = (double2*)e->dalloc(e->rho_bytes);
   e->rho_states = e->n_states;
   if (e->n_sites == 0) return;
+  // S sites per block (power of two, <= 64): all states of S + 1 sites in about 44 KB of LDS
   const size_t per_site = (size_t)e->n_states * e->n_orb * sizeof(double2);
-  int A = (int)std::min<size_t>(64, 49152 / per_site);
-  while (A > 1 && (size_t)(A + 1) * per_site > 49152) --A;
+  int S = 64;
+  while (S > 1 && (size_t)(S + 1) * per_site > 45056) S /= 2;
   e->kbegin();
-  if (A >= 1) {
-    const unsigned nb = (unsigned)((e->n_sites + A - 1) / A);
-    hipLaunchKernelGGL(k_products, dim3(nb), dim3(NT), (size_t)(A + 1) * per_site, e->stream, e->coeffs,
-                       e->d_site_atom, e->n_sites, e->n_atoms, e->n_states, e->n_orb, A, e->rho);
+  if ((size_t)(S + 1) * per_site <= 65536) {
+    const unsigned nb = (unsigned)((e->n_sites + S - 1) / S);
+    hipLaunchKernelGGL(k_products, dim3(nb), dim3(NT), (size_t)(S + 1) * per_site, e->stream, e->coeffs,
+                       e->n_sites, e->n_states, e->n_orb, S, e->rho);
   } else {
     hipLaunchKernelGGL(k_products_direct, dim3(grid1d((size_t)P * e->n_sites)), dim3(256), 0, e->stream,
-                       e->coeffs, e->d_site_atom, e->n_sites, e->n_atoms, e->n_states, e->n_orb, e->rho);
+                       e->coeffs, e->n_sites, e->n_states, e->n_orb, e->rho);
   }
   LAUNCH_CHECK();
   e->kend(K_PRODUCTS, (double)e->n_sites * (double)per_site + (double)P * e->n_sites * 16.0);
@@ -988,7 +989,7 @@ void free_atoms(clat_engine* e) {
   e->dfree(e->d_linemask, e->linemask_words * 4);
   e->linemask_words = 0;
   e->n_live = 0;
-  e->dfree(e->coeffs, (size_t)e->cap_states * e->n_atoms * e->n_orb * 16);
+  e->dfree(e->coeffs, std::max<size_t>(1, (size_t)e->cap_states * e->n_sites * e->n_orb * 16));
   e->dfree(e->rho, e->rho_bytes);
   e->rho_bytes = 0;
   e->rho_states = -1;
@@ -1456,16 +1457,28 @@ int clat_add_state(clat_engine* e, const double* coeffs) {
   if (!e || !coeffs) return set_err(CLAT_EINVAL, "invalid arguments");
   if (e->n_orb <= 0) return set_err(CLAT_ESTATE, "atoms have not been set");
   return guarded([&] {
-    const size_t per = (size_t)e->n_atoms * e->n_orb * sizeof(double2);
+    // stored in site order (the effective atom of every occupied cell)
+    const size_t per = (size_t)e->n_sites * e->n_orb * sizeof(double2);
+    const size_t in_bytes = (size_t)e->n_atoms * e->n_orb * sizeof(double2);
     if (e->n_states == e->cap_states) {
       const int ncap = std::max(4, 2 * e->cap_states);
-      double2* nc = (double2*)e->dalloc(per * ncap);
+      double2* nc = (double2*)e->dalloc(std::max<size_t>(1, per * ncap));
       if (e->n_states) HIPX(hipMemcpy(nc, e->coeffs, per * e->n_states, hipMemcpyDeviceToDevice));
-      e->dfree(e->coeffs, per * e->cap_states);
+      e->dfree(e->coeffs, std::max<size_t>(1, per * e->cap_states));
       e->coeffs = nc;
       e->cap_states = ncap;
     }
-    HIPX(hipMemcpy((char*)e->coeffs + per * e->n_states, coeffs, per, hipMemcpyHostToDevice));
+    double2* stage = (double2*)e->dalloc(std::max<size_t>(1, in_bytes));
+    hipError_t ce = hipMemcpyAsync(stage, coeffs, in_bytes, hipMemcpyHostToDevice, e->stream);
+    if (ce == hipSuccess && e->n_sites > 0)
+      hipLaunchKernelGGL(k_coeffs_to_sites, dim3(grid1d((size_t)e->n_sites * e->n_orb)), dim3(256), 0, e->stream,
+                         stage, e->d_site_atom, e->n_sites, e->n_orb, e->coeffs + (size_t)e->n_states * e->n_sites * e->n_orb);
+    hipError_t le = hipGetLastError();
+    hipError_t se = hipStreamSynchronize(e->stream);
+    e->dfree(stage, std::max<size_t>(1, in_bytes));
+    HIPX(ce);
+    HIPX(le);
+    HIPX(se);
     e->n_states += 1;
     e->rho_states = -1;
   });
@@ -1507,6 +1520,9 @@ int clat_compute_integrals(clat_engine* e, const int32_t* quads, int64_t n, int 
       prepare_sites(e, e->rho + (size_t)slots[(size_t)sl] * e->n_sites, e->Vat + (size_t)sl * e->n_sites);
     HIPX(hipEventRecord(e->stage_ev[3], e->stream));
     // overlaps of every (slot, product) pair
+    // tile edge 3 TX with TX x TX threads (TX = 16 measured fastest for tens of products)
+    constexpr int TX = 16;
+    const int OV_T = 3 * TX;
     const int tl = (nL + OV_T - 1) / OV_T, tr = (P + OV_T - 1) / OV_T;
     int64_t n_chunks = std::max<int64_t>(1, 2048 / std::max(1, tl * tr));
     n_chunks = std::min<int64_t>(n_chunks, (e->n_sites + OV_S - 1) / OV_S);
@@ -1528,8 +1544,9 @@ int clat_compute_integrals(clat_engine* e, const int32_t* quads, int64_t n, int 
     std::vector<double> ov(n_out, 0.0);
     if (nL > 0 && e->n_sites > 0) {
       e->kbegin();
-      hipLaunchKernelGGL(k_overlap_partial, dim3(tl, tr, (unsigned)n_chunks), dim3(NT), 0, e->stream, e->Vat,
-                         e->rho, nL, P, e->n_sites, chunk, e->part);
+      const dim3 og(tl, tr, (unsigned)n_chunks);
+      hipLaunchKernelGGL(k_overlap_partial<TX>, og, dim3(TX * TX), 0, e->stream, e->Vat, e->rho, nL, P, e->n_sites,
+                         chunk, e->part);
       LAUNCH_CHECK();
       hipLaunchKernelGGL(k_overlap_reduce, dim3((unsigned)((n_out + 63) / 64)), dim3(1024), 0, e->stream, e->part,
                          n_chunks, (int64_t)n_out, e->ovl);

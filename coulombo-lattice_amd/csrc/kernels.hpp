@@ -444,72 +444,86 @@ __global__ void k_gather(const C2<T>* __restrict__ V, const int32_t* __restrict_
 }
 
 // ---------------------------------------------------------------------------
-// Pair densities: rho_P[k] = sum_o conj(c_L[a,o]) c_R[a,o] for every product
+// Pair densities: rho_P[k] = sum_o conj(c_L[k,o]) c_R[k,o] for every product
 // P = L(L+1)/2 + R, R <= L (FunctionCollection::createProducts,
 // FunctionCollection.cpp:91-105; ProductFromTightBinding::map, Product.cpp:16-30).
-// coeffs: [state][atom][orbital]; atom[k]: atom of effective site k.
-// A block stages A sites x all states in LDS.
+// coeffs: [state][site][orbital] (site order, gathered at upload).
+// A block stages S sites x all states in LDS as [state][orbital][site] (pitch
+// S + 1: conflict-free 16-byte accesses), G = blockDim / S thread groups
+// share the products of each site.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(NT) void k_products(const double2* __restrict__ coeffs,
-                                                 const int32_t* __restrict__ atom, int64_t n_sites,
-                                                 int64_t n_atoms, int n_states, int n_orb, int A,
-                                                 double2* __restrict__ rho) {
+__device__ __forceinline__ void pair_of(int p, int& L, int& R) {
+  L = (int)((sqrt(8.0 * p + 1.0) - 1.0) * 0.5);
+  while (L * (L + 1) / 2 > p) --L;
+  while ((L + 1) * (L + 2) / 2 <= p) ++L;
+  R = p - L * (L + 1) / 2;
+}
+
+__global__ __launch_bounds__(NT) void k_products(const double2* __restrict__ coeffs, int64_t n_sites,
+                                                 int n_states, int n_orb, int S, double2* __restrict__ rho) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
-  double2* cs = reinterpret_cast<double2*>(smem_raw);  // [s][o][a], pitch AP per orbital row
-  const int AP = A + 1;
-  const int64_t k0 = (int64_t)blockIdx.x * A;
-  const int nA = (int)min((int64_t)A, n_sites - k0);
-  const int per = A * n_orb;
-  const int tot = n_states * per;
-  for (int e = threadIdx.x; e < tot; e += NT) {
-    // global order: state, site, orbital (orbital fastest: contiguous rows)
-    const int s = e / per;
-    const int r = e - s * per;
-    const int a = r / n_orb;
-    const int o = r - a * n_orb;
-    double2 v = make_double2(0.0, 0.0);
-    if (a < nA) v = coeffs[((size_t)s * n_atoms + atom[k0 + a]) * n_orb + o];
-    cs[((size_t)s * n_orb + o) * AP + a] = v;
+  double2* cs = reinterpret_cast<double2*>(smem_raw);
+  const int SP = S + 1;
+  const int64_t k0 = (int64_t)blockIdx.x * S;
+  const int nA = (int)min((int64_t)S, n_sites - k0);
+  const int per = nA * n_orb;  // contiguous elements of one state for this block
+  // element w of every state's chunk: one division per element, the state
+  // loop unrolled so that several loads are in flight before their LDS writes
+  for (int w = threadIdx.x; w < per; w += NT) {
+    const int a = w / n_orb, o = w - a * n_orb;
+    const double2* src = coeffs + (size_t)k0 * n_orb + w;
+    double2* dst = cs + (size_t)o * SP + a;
+    int s = 0;
+    for (; s + 4 <= n_states; s += 4) {
+      double2 v[4];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) v[j] = src[(size_t)(s + j) * n_sites * n_orb];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) dst[(size_t)(s + j) * n_orb * SP] = v[j];
+    }
+    for (; s < n_states; ++s) dst[(size_t)s * n_orb * SP] = src[(size_t)s * n_sites * n_orb];
   }
   __syncthreads();
+  const int a = threadIdx.x % S, g = threadIdx.x / S, G = NT / S;
+  if (a >= nA) return;
   const int P = n_states * (n_states + 1) / 2;
-  const int tot2 = P * A;
-  for (int e = threadIdx.x; e < tot2; e += NT) {
-    const int pidx = e / A;
-    const int a = e - pidx * A;
-    if (a >= nA) continue;
-    // pidx -> (fL, fR) with pidx = fL(fL+1)/2 + fR
-    int fL = (int)((sqrt(8.0 * pidx + 1.0) - 1.0) * 0.5);
-    while (fL * (fL + 1) / 2 > pidx) --fL;
-    while ((fL + 1) * (fL + 2) / 2 <= pidx) ++fL;
-    const int fR = pidx - fL * (fL + 1) / 2;
-    const double2* l = cs + (size_t)fL * n_orb * AP + a;
-    const double2* r = cs + (size_t)fR * n_orb * AP + a;
+  for (int p = g; p < P; p += G) {
+    int L, R;
+    pair_of(p, L, R);
+    const double2* l = cs + (size_t)L * n_orb * SP + a;
+    const double2* r = cs + (size_t)R * n_orb * SP + a;
     double re = 0.0, im = 0.0;
     for (int o = 0; o < n_orb; ++o) {
-      const double2 x = l[(size_t)o * AP], y = r[(size_t)o * AP];
+      const double2 x = l[(size_t)o * SP], y = r[(size_t)o * SP];
       re += x.x * y.x + x.y * y.y;
       im += x.x * y.y - x.y * y.x;
     }
-    rho[(size_t)pidx * n_sites + k0 + a] = make_double2(re, im);
+    rho[(size_t)p * n_sites + k0 + a] = make_double2(re, im);
+  }
+}
+
+// out[(k*n_orb + o)] = in[(atom[k]*n_orb + o)]: one state's coefficients in site order
+__global__ void k_coeffs_to_sites(const double2* __restrict__ in, const int32_t* __restrict__ atom,
+                                  int64_t n_sites, int n_orb, double2* __restrict__ out) {
+  const int64_t n = n_sites * n_orb;
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t k = t / n_orb;
+    out[t] = in[(int64_t)atom[k] * n_orb + (t - k * n_orb)];
   }
 }
 
 // Same without LDS staging, for state counts whose staging would not fit.
-__global__ void k_products_direct(const double2* __restrict__ coeffs, const int32_t* __restrict__ atom,
-                                  int64_t n_sites, int64_t n_atoms, int n_states, int n_orb,
+__global__ void k_products_direct(const double2* __restrict__ coeffs, int64_t n_sites, int n_states, int n_orb,
                                   double2* __restrict__ rho) {
   const int P = n_states * (n_states + 1) / 2;
   const int64_t tot = (int64_t)P * n_sites;
   for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < tot; t += (int64_t)gridDim.x * blockDim.x) {
     const int pidx = (int)(t / n_sites);
     const int64_t k = t - (int64_t)pidx * n_sites;
-    int fL = (int)((sqrt(8.0 * pidx + 1.0) - 1.0) * 0.5);
-    while (fL * (fL + 1) / 2 > pidx) --fL;
-    while ((fL + 1) * (fL + 2) / 2 <= pidx) ++fL;
-    const int fR = pidx - fL * (fL + 1) / 2;
-    const double2* l = coeffs + ((size_t)fL * n_atoms + atom[k]) * n_orb;
-    const double2* r = coeffs + ((size_t)fR * n_atoms + atom[k]) * n_orb;
+    int fL, fR;
+    pair_of(pidx, fL, fR);
+    const double2* l = coeffs + ((size_t)fL * n_sites + k) * n_orb;
+    const double2* r = coeffs + ((size_t)fR * n_sites + k) * n_orb;
     double re = 0.0, im = 0.0;
     for (int o = 0; o < n_orb; ++o) {
       const double2 x = l[o], y = r[o];
@@ -529,17 +543,19 @@ __global__ void k_products_direct(const double2* __restrict__ coeffs, const int3
 // Stage 1: a 48x48 (slot, product) tile per block, 3x3 pairs per thread, over
 // one chunk of sites; partial sums to part[chunk][l][r][4].
 // ---------------------------------------------------------------------------
-constexpr int OV_T = 48;   // tile edge
 constexpr int OV_S = 32;   // sites staged per step
 
-__global__ __launch_bounds__(NT) void k_overlap_partial(const double2* __restrict__ Vat,
-                                                        const double2* __restrict__ rho,
-                                                        int n_left, int n_right, int64_t n_sites,
-                                                        int64_t chunk, double* __restrict__ part) {
-  __shared__ double2 sv[OV_T][OV_S + 1];
-  __shared__ double2 sr[OV_T][OV_S + 1];
-  const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
-  const int l0 = blockIdx.x * OV_T, r0 = blockIdx.y * OV_T;
+// TX x TX threads, 3 x 3 (left, right) pairs each: a tile edge of 3 TX.
+template <int TX>
+__global__ __launch_bounds__(TX * TX) void k_overlap_partial(const double2* __restrict__ Vat,
+                                                             const double2* __restrict__ rho,
+                                                             int n_left, int n_right, int64_t n_sites,
+                                                             int64_t chunk, double* __restrict__ part) {
+  constexpr int T = 3 * TX, NTH = TX * TX;
+  __shared__ double2 sv[T][OV_S + 1];
+  __shared__ double2 sr[T][OV_S + 1];
+  const int tx = threadIdx.x % TX, ty = threadIdx.x / TX;
+  const int l0 = blockIdx.x * T, r0 = blockIdx.y * T;
   const int64_t s0 = (int64_t)blockIdx.z * chunk;
   const int64_t s1 = min(n_sites, s0 + chunk);
   double acc[3][3][4];
@@ -549,20 +565,33 @@ __global__ __launch_bounds__(NT) void k_overlap_partial(const double2* __restric
     for (int j = 0; j < 3; ++j)
 #pragma unroll
       for (int q = 0; q < 4; ++q) acc[i][j][q] = 0.0;
+  constexpr int NE = (T * OV_S + NTH - 1) / NTH;  // staged elements per thread and table
   for (int64_t sb = s0; sb < s1; sb += OV_S) {
-    for (int e = threadIdx.x; e < OV_T * OV_S; e += NT) {
+    double2 bv[NE], br[NE];
+#pragma unroll
+    for (int j = 0; j < NE; ++j) {
+      const int e = threadIdx.x + j * NTH;
       const int row = e / OV_S, col = e - row * OV_S;
       const int64_t site = sb + col;
-      const bool ok = site < s1;
-      sv[row][col] = (ok && l0 + row < n_left) ? Vat[(size_t)(l0 + row) * n_sites + site] : make_double2(0.0, 0.0);
-      sr[row][col] = (ok && r0 + row < n_right) ? rho[(size_t)(r0 + row) * n_sites + site] : make_double2(0.0, 0.0);
+      const bool ok = e < T * OV_S && site < s1;
+      bv[j] = (ok && l0 + row < n_left) ? Vat[(size_t)(l0 + row) * n_sites + site] : make_double2(0.0, 0.0);
+      br[j] = (ok && r0 + row < n_right) ? rho[(size_t)(r0 + row) * n_sites + site] : make_double2(0.0, 0.0);
+    }
+#pragma unroll
+    for (int j = 0; j < NE; ++j) {
+      const int e = threadIdx.x + j * NTH;
+      if (e < T * OV_S) {
+        const int row = e / OV_S, col = e - row * OV_S;
+        sv[row][col] = bv[j];
+        sr[row][col] = br[j];
+      }
     }
     __syncthreads();
 #pragma unroll 4
     for (int s = 0; s < OV_S; ++s) {
       double2 v[3], r[3];
 #pragma unroll
-      for (int i = 0; i < 3; ++i) { v[i] = sv[ty + 16 * i][s]; r[i] = sr[tx + 16 * i][s]; }
+      for (int i = 0; i < 3; ++i) { v[i] = sv[ty + TX * i][s]; r[i] = sr[tx + TX * i][s]; }
 #pragma unroll
       for (int i = 0; i < 3; ++i)
 #pragma unroll
@@ -579,7 +608,7 @@ __global__ __launch_bounds__(NT) void k_overlap_partial(const double2* __restric
   for (int i = 0; i < 3; ++i)
 #pragma unroll
     for (int j = 0; j < 3; ++j) {
-      const int l = l0 + ty + 16 * i, r = r0 + tx + 16 * j;
+      const int l = l0 + ty + TX * i, r = r0 + tx + TX * j;
       if (l < n_left && r < n_right) {
         double* d = part + (((size_t)blockIdx.z * n_left + l) * n_right + r) * 4;
 #pragma unroll
