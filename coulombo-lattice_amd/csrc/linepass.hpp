@@ -29,14 +29,14 @@ struct LineArgs {
 
 template <int C, int R1, int R2, int NSUB>
 constexpr int line_threads() {
-  constexpr int m = C * NSUB * R2 > C * R1 ? C * NSUB * R2 : C * R1;
+  constexpr int m = R2 > R1 ? C * NSUB * R2 : C * NSUB * R1;
   return (m + 63) / 64 * 64;
 }
-// lines per block: about 256 threads in the phase with NSUB R2 items per
-// line, half as many lines when the arithmetic is FP64
-template <typename TC, int R2, int NSUB>
+// lines per block: about 256 threads (NSUB R2 items per line in one phase,
+// NSUB R1 in the others), half as many lines when the arithmetic is FP64
+template <typename TC, int R1, int R2, int NSUB>
 constexpr int line_c() {
-  constexpr int q = (256 / (NSUB * R2)) / (sizeof(TC) == 8 ? 2 : 1);
+  constexpr int q = (256 / (NSUB * (R1 > R2 ? R1 : R2))) / (sizeof(TC) == 8 ? 2 : 1);
   return q >= 16 ? 16 : q >= 8 ? 8 : q >= 4 ? 4 : q >= 2 ? 2 : 1;
 }
 template <int R1>
@@ -67,9 +67,9 @@ __global__ __launch_bounds__((line_threads<C, R1, R2, NSUB>())) void k_linepass(
   auto xi = [](int c, int s, int k2, int n1) { return ((c * NSUB + s) * R2 + k2) * R1P + n1; };
 
 
-  // (c, n1) items of the phases holding R2 values
-  const int c13 = tid / R1, n1 = tid - c13 * R1;
-  const bool ok13 = tid < C * R1 && c13 < nC;
+  // (c, s, n1) items of the phases holding R2 values: one parity each
+  const int n1 = tid % R1, s13 = (tid / R1) % NSUB, c13 = tid / (R1 * NSUB);
+  const bool ok13 = tid < C * NSUB * R1 && c13 < nC;
   // (c, s, k2) items of the phase holding R1 values
   const int s2 = tid & (NSUB - 1), k2 = (tid / NSUB) % R2, c2 = tid / (NSUB * R2);
   const bool ok2 = tid < C * NSUB * R2 && c2 < nC;
@@ -89,84 +89,79 @@ __global__ __launch_bounds__((line_threads<C, R1, R2, NSUB>())) void k_linepass(
       lds[a.site_ccell[k] - cell0] = mk<TC>((TC)v.x, (TC)v.y);
     }
     __syncthreads();
-    CT x[R2];
+    CT v[R2];
 #pragma unroll
-    for (int n2 = 0; n2 < R2; ++n2) x[n2] = ok13 ? lds[c13 * L + n1 + R1 * n2] : zero;
-    __syncthreads();
+    for (int n2 = 0; n2 < R2; ++n2) v[n2] = ok13 ? lds[c13 * L + n1 + R1 * n2] : zero;
+    __syncthreads();  // the line buffer and the exchange share LDS
     // ---------------------------------------------------------- P1
     if (ok13) {
+      if (s13 == 1) rot_all<R2, false, TC>(v);
+      rdft<R2, false, TC>(v);
+      const uint32_t tstep = (uint32_t)(2 * n1) * SC, tbase = (uint32_t)(s13 * n1) * SC;
 #pragma unroll
-      for (int s = 0; s < NSUB; ++s) {
-        CT v[R2];
-#pragma unroll
-        for (int n2 = 0; n2 < R2; ++n2) v[n2] = x[n2];
-        if (s == 1) rot_all<R2, false, TC>(v);
-        rdft<R2, false, TC>(v);
-        const uint32_t tstep = (uint32_t)(2 * n1) * SC, tbase = (uint32_t)(s * n1) * SC;
-#pragma unroll
-        for (int q = 0; q < R2; ++q) {
-          CT t = v[q];
-          if (q > 0 || s == 1) t = cmul<TC>(t, TW(tbase + (uint32_t)q * tstep));
-          lds[xi(c13, s, q, n1)] = t;
-        }
+      for (int q = 0; q < R2; ++q) {
+        CT t = v[q];
+        if (q > 0 || s13 == 1) t = cmul<TC>(t, TW(tbase + (uint32_t)q * tstep));
+        lds[xi(c13, s13, q, n1)] = t;
       }
     }
     __syncthreads();
     // ---------------------------------------------------------- P2 -> A
     if (ok2) {
-      CT v[R1];
+      CT w[R1];
 #pragma unroll
-      for (int m = 0; m < R1; ++m) v[m] = lds[xi(c2, s2, k2, m)];
-      rdft<R1, false, TC>(v);
+      for (int m = 0; m < R1; ++m) w[m] = lds[xi(c2, s2, k2, m)];
+      rdft<R1, false, TC>(w);
       C2<TO>* o = out + (size_t)a.lines[b0 + c2] * a.out_ls + k2 * NSUB + s2;
 #pragma unroll
-      for (int k1 = 0; k1 < R1; ++k1) o[k1 * R2 * NSUB] = cvt<TC, TO>(v[k1]);
+      for (int k1 = 0; k1 < R1; ++k1) o[k1 * R2 * NSUB] = cvt<TC, TO>(w[k1]);
     }
   } else {
     // ---------------------------------------------------------- P2 <- D
     // (the twiddles come from global memory here: a barrier before this phase
     // costs more than their loads)
-    CT v[R1];
     if (ok2) {
+      CT w[R1];
       const C2<TI>* ip = in + (size_t)a.lines[b0 + c2] * a.in_ls + k2 * NSUB + s2;
 #pragma unroll
-      for (int k1 = 0; k1 < R1; ++k1) v[k1] = cvt<TI, TC>(ip[k1 * R2 * NSUB]);
-      rdft<R1, true, TC>(v);
+      for (int k1 = 0; k1 < R1; ++k1) w[k1] = cvt<TI, TC>(ip[k1 * R2 * NSUB]);
+      rdft<R1, true, TC>(w);
       const uint32_t tstep = (uint32_t)(2 * k2 + s2) * SC;
 #pragma unroll
       for (int m = 0; m < R1; ++m) {
-        CT t = v[m];
+        CT t = w[m];
         if (m > 0) t = cmulc<TC>(t, ldg_off(T2L, (uint32_t)m * tstep));
         lds[xi(c2, s2, k2, m)] = t;
       }
     }
     __syncthreads();
-    // ---------------------------------------------------------- P3
-    CT acc[R2];
-#pragma unroll
-    for (int n2 = 0; n2 < R2; ++n2) acc[n2] = zero;
+    // ---------------------------------------------------------- P3: one parity per item
+    CT v[R2];
     if (ok13) {
 #pragma unroll
-      for (int s = 0; s < NSUB; ++s) {
-        CT v[R2];
-#pragma unroll
-        for (int q = 0; q < R2; ++q) v[q] = lds[xi(c13, s, q, n1)];
-        rdft<R2, true, TC>(v);
-        if (s == 1) rot_all<R2, true, TC>(v);
-#pragma unroll
-        for (int n2 = 0; n2 < R2; ++n2) acc[n2] = cadd<TC>(acc[n2], v[n2]);
-      }
+      for (int q = 0; q < R2; ++q) v[q] = lds[xi(c13, s13, q, n1)];
+      rdft<R2, true, TC>(v);
+      if (s13 == 1) rot_all<R2, true, TC>(v);
     }
     __syncthreads();
+    // both parities of each line side by side; the gather adds them
     if (ok13) {
 #pragma unroll
-      for (int n2 = 0; n2 < R2; ++n2) lds[c13 * L + n1 + R1 * n2] = acc[n2];
+      for (int n2 = 0; n2 < R2; ++n2) lds[(c13 * NSUB + s13) * L + n1 + R1 * n2] = v[n2];
     }
     __syncthreads();
     // ---------------------------------------------------------- gather
     for (int k = s0 + tid; k < s1; k += NTH) {
-      const CT v = lds[a.site_ccell[k] - cell0];
-      a.site_out[k] = make_double2((double)v.x, (double)v.y);
+      const int cc = a.site_ccell[k] - cell0;
+      const int c = cc / L, x = cc - c * L;
+      const CT u = lds[(c * NSUB) * L + x];
+      double2 r = make_double2((double)u.x, (double)u.y);
+      if (NSUB == 2) {
+        const CT u1 = lds[(c * NSUB + 1) * L + x];
+        r.x += (double)u1.x;
+        r.y += (double)u1.y;
+      }
+      a.site_out[k] = r;
     }
   }
 }
@@ -183,7 +178,7 @@ struct LineLaunch {
 template <typename TI, typename TC, typename TO, int MODE, int R1, int R2>
 hipError_t line_go(const LineLaunch& L) {
   constexpr int NSUB = 2;
-  constexpr int C = line_c<TC, R2, NSUB>();
+  constexpr int C = line_c<TC, R1, R2, NSUB>();
   constexpr int nth = line_threads<C, R1, R2, NSUB>();
   constexpr size_t lds = line_lds<TC, R1, R2, NSUB, C>();
   static_assert(lds <= 65536, "line pass tile exceeds 64 KiB of LDS");
@@ -195,7 +190,7 @@ hipError_t line_go(const LineLaunch& L) {
 // lines per block of a pair, for the host's grid size
 template <int R1, int R2>
 int line_lines(bool fp64_arith) {
-  return fp64_arith ? line_c<double, R2, 2>() : line_c<float, R2, 2>();
+  return fp64_arith ? line_c<double, R1, R2, 2>() : line_c<float, R1, R2, 2>();
 }
 
 // (types, mode) pairs of the atom-path x passes; hipErrorInvalidValue otherwise
