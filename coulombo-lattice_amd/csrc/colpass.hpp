@@ -24,6 +24,7 @@ struct ColArgs2 {
   //   (o / od) * os_hi + (o % od) * os_lo
   int od;
   long long in_os_hi, in_os_lo, out_os_hi, out_os_lo;
+  long long in_bs, out_bs;        // element stride between batched prepares (blockIdx.z)
   // rows: n-domain row n1 + R1 n2 -> byte offset n2 * is_hi + n1 * is_lo;
   // k-domain rows j (y passes only) -> j * is_lo
   unsigned in_is_lo, in_is_hi, out_is_lo, out_is_hi;
@@ -108,8 +109,10 @@ __global__ __launch_bounds__((col_threads<C, R1>())) void k_colpass(const C2<TI>
   }
   const int oh = o / a.od, ol = o - oh * a.od;
   // per-block bases; every later offset is 32-bit
-  const C2<TI>* bin = in + (size_t)oh * a.in_os_hi + (size_t)ol * a.in_os_lo + (size_t)blockIdx.x * C;
-  C2<TO>* bout = out + (size_t)oh * a.out_os_hi + (size_t)ol * a.out_os_lo + (size_t)blockIdx.x * C;
+  const C2<TI>* bin = in + (size_t)blockIdx.z * a.in_bs + (size_t)oh * a.in_os_hi + (size_t)ol * a.in_os_lo +
+                      (size_t)blockIdx.x * C;
+  C2<TO>* bout = out + (size_t)blockIdx.z * a.out_bs + (size_t)oh * a.out_os_hi + (size_t)ol * a.out_os_lo +
+                 (size_t)blockIdx.x * C;
   const uint32_t cin = (uint32_t)c * SI, cout = (uint32_t)c * SO;
 
   // pass twiddles W_2L^m, m < 2L, go to LDS: read there by every phase, they

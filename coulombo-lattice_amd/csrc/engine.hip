@@ -259,6 +259,11 @@ struct clat_engine {
   int32_t* d_site_ccell = nullptr;   // occupied-line index * Xi + x of each site
   uint32_t* d_linemask = nullptr;    // y passes: word (z, n1), bit n2 = line y = n1 + R1y n2 holds a site
   size_t linemask_words = 0;
+  // batched prepares of the atom path (line kernels): up to nb_max per launch
+  int nb_max = 1, cur_nb = 1;
+  const double2* b_in[LINE_NB] = {};
+  double2* b_out[LINE_NB] = {};
+  size_t a_set = 0, b_set = 0, d_set = 0;  // elements of A, B, D per prepare
   bool xline = false;                // x passes on the line kernel, y passes skip empty lines
   int n_states = 0, cap_states = 0;
   double2* coeffs = nullptr;
@@ -398,7 +403,7 @@ bool launch_col(clat_engine* e, const PassCfg& pc, const P2Args& a, int n_outer,
   }
   const int C = col_c<TC>();
   ColLaunch L;
-  L.grid = dim3((unsigned)((a.W + C - 1) / C), (unsigned)n_outer);
+  L.grid = dim3((unsigned)((a.W + C - 1) / C), (unsigned)n_outer, (unsigned)std::max(1, a.nb));
   L.stream = e->stream;
   L.in = in;
   L.out = out;
@@ -417,6 +422,8 @@ bool launch_col(clat_engine* e, const PassCfg& pc, const P2Args& a, int n_outer,
   c.pair = (MODE == PM_CONV && e->gc_npar == 1) ? 1 : 0;
   c.rowmask = a.rowmask;
   c.mask_os = a.mask_os;
+  c.in_bs = a.in_bs;
+  c.out_bs = a.out_bs;
   L.T2L = sizeof(TC) == 8 ? (const void*)e->dtw[axis] : (const void*)e->dtw32[axis];
   L.G = gc_of<TC>(e);
   constexpr int combo = combo_of<TI, TC, TO>();
@@ -440,7 +447,7 @@ void launch2(clat_engine* e, const PassCfg& pc, P2Args a, int n_outer, int axis,
   if constexpr (!LINE) {
     if (launch_col<TI, TC, TO, MODE>(e, pc, a, n_outer, axis, in, out, kind, bytes)) return;
   }
-  if (a.rowmask) throw host::Error(CLAT_EDEVICE, "row mask on the general pass kernel");
+  if (a.rowmask || a.nb > 1) throw host::Error(CLAT_EDEVICE, "row mask or batch on the general pass kernel");
   a.R1 = pc.R1;
   a.R2 = pc.R2;
   a.R1p = pc.R1p;
@@ -477,21 +484,26 @@ void launch2(clat_engine* e, const PassCfg& pc, P2Args a, int n_outer, int axis,
 }
 
 // x pass of the atom path on the line kernel (occupied lines only)
+// in / out: dense side of prepare 0, bs: its element stride to the next
+// prepare of the batch (e->cur_nb prepares, sites in e->b_in / e->b_out)
 template <typename TI, typename TC, typename TO, int MODE>
-void launch_line(clat_engine* e, const void* in, void* out, const double2* sites_in, double2* sites_out, int kind,
-                 double bytes) {
+void launch_line(clat_engine* e, const void* in, void* out, long long bs, int kind, double bytes) {
   LineLaunch L;
   L.stream = e->stream;
   L.in = in;
   L.out = out;
+  L.nb = (unsigned)e->cur_nb;
   LineArgs& a = L.a;
   a.n_live = (int)e->n_live;
   a.lines = e->d_lines;
   a.live_ptr = e->d_live_ptr;
   a.site_ccell = e->d_site_ccell;
-  a.site_in = sites_in;
-  a.site_out = sites_out;
+  for (int b = 0; b < LINE_NB; ++b) {
+    a.site_in[b] = MODE == PM_FWD ? e->b_in[b] : nullptr;
+    a.site_out[b] = MODE == PM_INV ? e->b_out[b] : nullptr;
+  }
   a.in_ls = a.out_ls = 2LL * e->Xi;
+  a.in_bs = a.out_bs = bs;
   L.T2L = sizeof(TC) == 8 ? (const void*)e->dtw[0] : (const void*)e->dtw32[0];
   constexpr int combo = combo_of<TI, TC, TO>();
   const int R1 = e->cxf_s.R1, R2 = e->cxf_s.R2;
@@ -548,6 +560,17 @@ void prepare_fast(clat_engine* e, const double2* sites_in, double2* sites_out) {
   // passes never touch the rows of empty lines
   const bool lk = atoms && e->shared() && e->xline;
   const double live = lk ? (double)e->n_live / (double)nlines : 1.0;
+  // batched prepares (line-kernel path only): e->cur_nb of them, sites in e->b_in / e->b_out
+  const int nb = lk ? e->cur_nb : 1;
+  if (nb == 1) {
+    e->b_in[0] = sites_in;
+    e->b_out[0] = sites_out;
+  }
+  auto batch = [&](P2Args& a, long long in_bs, long long out_bs) {
+    a.nb = nb;
+    a.in_bs = in_bs;
+    a.out_bs = out_bs;
+  };
 
   auto xfwd = [&](int nsub, int kx, int ky, int kz, void* out) {
     P2Args a = base_args(e);
@@ -560,7 +583,7 @@ void prepare_fast(clat_engine* e, const double2* sites_in, double2* sites_out) {
     a.kx = kx; a.ky = ky; a.kz = kz;
     const double bytes = (atoms ? site_bytes : si * N) + sa * N * nsub * live;
     if (lk) {
-      launch_line<double, TC, TA, PM_FWD>(e, nullptr, out, sites_in, nullptr, K_XFWD, bytes);
+      launch_line<double, TC, TA, PM_FWD>(e, nullptr, out, (long long)e->a_set, K_XFWD, bytes * nb);
     } else if (atoms) {
       a.site_ptr = e->d_site_ptr;
       a.site_cell = e->d_site_cell;
@@ -581,8 +604,8 @@ void prepare_fast(clat_engine* e, const double2* sites_in, double2* sites_out) {
     a.kx = kx; a.ky = ky; a.kz = kz;
     a.acc = acc;
     if (lk) {
-      launch_line<TD, TCX, double, PM_INV>(e, in, nullptr, nullptr, sites_out, K_XINV,
-                                           sd * N * nsub * live + site_bytes);
+      launch_line<TD, TCX, double, PM_INV>(e, in, nullptr, (long long)e->d_set, K_XINV,
+                                           (sd * N * nsub * live + site_bytes) * nb);
     } else if (atoms) {
       a.site_ptr = e->d_site_ptr;
       a.site_cell = e->d_site_cell;
@@ -603,23 +626,26 @@ void prepare_fast(clat_engine* e, const double2* sites_in, double2* sites_out) {
       P2Args a = base_args(e);
       a.L = Yi; a.nsub = 2; a.W = (int)W;
       a.in_os = (long long)Yi * W; a.in_is = W; a.out_os = W; a.out_is = (long long)Zi * W;
-      if (lk) { a.rowmask = e->d_linemask; a.mask_os = e->cyf.R1; }
-      launch2<TA, TC, TB, PM_FWD, false>(e, e->cyf, a, Zi, 1, e->A, e->B, K_YFWD, (sa * 2 * live + sb * 4) * N);
+      if (lk) { a.rowmask = e->d_linemask; a.mask_os = e->cyf.R1; batch(a, (long long)e->a_set, (long long)e->b_set); }
+      launch2<TA, TC, TB, PM_FWD, false>(e, e->cyf, a, Zi, 1, e->A, e->B, K_YFWD,
+                                         (sa * 2 * live + sb * 4) * N * nb);
     }
     {
       P2Args a = base_args(e);
       a.L = Zi; a.nsub = 2; a.W = (int)W;
       a.in_os = a.out_os = (long long)Zi * W; a.in_is = a.out_is = W;
       a.gxm = 1; a.gym = 1;
+      if (lk) batch(a, (long long)e->b_set, (long long)e->b_set);
       launch2<TB, TCZ, TB, PM_CONV, false>(e, e->czc, a, 2 * Yi, 2, e->B, e->B, K_ZCONV,
-                                           sb * 8 * N + (double)sizeof(TCZ) * (double)e->NG());
+                                           (sb * 8 * N + (double)sizeof(TCZ) * (double)e->NG()) * nb);
     }
     {
       P2Args a = base_args(e);
       a.L = Yi; a.nsub = 2; a.W = (int)W;
       a.in_os = W; a.in_is = (long long)Zi * W; a.out_os = (long long)Yi * W; a.out_is = W;
-      if (lk) { a.rowmask = e->d_linemask; a.mask_os = e->cyi.R1; }
-      launch2<TB, TCY, TD, PM_INV, false>(e, e->cyi, a, Zi, 1, e->B, e->D, K_YINV, (sb * 4 + sd * 2 * live) * N);
+      if (lk) { a.rowmask = e->d_linemask; a.mask_os = e->cyi.R1; batch(a, (long long)e->b_set, (long long)e->d_set); }
+      launch2<TB, TCY, TD, PM_INV, false>(e, e->cyi, a, Zi, 1, e->B, e->D, K_YINV,
+                                          (sb * 4 + sd * 2 * live) * N * nb);
     }
     xinv(2, 0, 0, 0, 0, e->D);
   } else {
@@ -1210,16 +1236,26 @@ int clat_create(int X, int Y, int Z, int precision, int schedule, clat_engine** 
     const size_t N = e->N();
     e->f_bytes = N * e->s_in;
     e->v_bytes = N * e->s_v;
-    e->a_bytes = (e->shared() ? 2 : 1) * N * e->s_a;
+    e->a_set = (e->shared() ? 2 : 1) * N;
+    e->b_set = e->shared() ? 4 * N : 0;
+    e->d_set = e->a_set;
+    // the atom path batches up to four prepares per launch (all five passes on
+    // the compile-time kernels) while the buffer sets stay under 16 GB
+    const size_t set_bytes = e->a_set * e->s_a + e->b_set * e->s_b + (e->s_d == e->s_a ? 0 : e->d_set * e->s_d);
+    e->nb_max = 1;
+    if (e->xline && e->conv_tiled)
+      while (e->nb_max < LINE_NB && (size_t)(2 * e->nb_max) * set_bytes <= ((size_t)16 << 30)) e->nb_max *= 2;
+    if (const char* g = std::getenv("CLAT_BATCH")) e->nb_max = std::max(1, std::min(e->nb_max, std::atoi(g)));
+    e->a_bytes = e->nb_max * e->a_set * e->s_a;
     e->A = e->dalloc(e->a_bytes);
     if (e->shared()) {
-      e->b_bytes = 4 * N * e->s_b;
+      e->b_bytes = e->nb_max * e->b_set * e->s_b;
       e->B = e->dalloc(e->b_bytes);
     }
     if (e->s_d == e->s_a) {
       e->D = e->A;  // the y-inverse output reuses the x-forward buffer
     } else {
-      e->d_bytes = (e->shared() ? 2 : 1) * N * e->s_d;
+      e->d_bytes = e->nb_max * e->d_set * e->s_d;
       e->D = e->dalloc(e->d_bytes);
     }
     HIPX(hipStreamSynchronize(e->stream));
@@ -1516,8 +1552,17 @@ int clat_compute_integrals(clat_engine* e, const int32_t* quads, int64_t n, int 
       e->Vat = (double2*)e->dalloc(vat_need * sizeof(double2));
       e->vat_cap = vat_need;
     }
-    for (int sl = 0; sl < nL && e->n_sites > 0; ++sl)
-      prepare_sites(e, e->rho + (size_t)slots[(size_t)sl] * e->n_sites, e->Vat + (size_t)sl * e->n_sites);
+    for (int sl = 0; sl < nL && e->n_sites > 0;) {
+      const int nb = std::min(e->nb_max, nL - sl);
+      e->cur_nb = nb;
+      for (int b = 0; b < nb; ++b) {
+        e->b_in[b] = e->rho + (size_t)slots[(size_t)(sl + b)] * e->n_sites;
+        e->b_out[b] = e->Vat + (size_t)(sl + b) * e->n_sites;
+      }
+      prepare_sites(e, e->b_in[0], e->b_out[0]);
+      e->cur_nb = 1;
+      sl += nb;
+    }
     HIPX(hipEventRecord(e->stage_ev[3], e->stream));
     // overlaps of every (slot, product) pair
     // tile edge 3 TX with TX x TX threads (TX = 16 measured fastest for tens of products)

@@ -17,14 +17,18 @@
 
 namespace clat {
 
+constexpr int LINE_NB = 4;  // prepares one launch can carry
+
 struct LineArgs {
   int n_live;                   // occupied lines
   const int32_t* lines;         // line id of each compact index
   const int32_t* live_ptr;      // CSR of the sites over compact lines (n_live + 1)
   const int32_t* site_ccell;    // compact cell of each site: compact line * L + x
-  const double2* site_in;       // forward: pair density at the sites
-  double2* site_out;            // inverse: potential at the sites
-  long long in_ls, out_ls;      // element stride between lines of the dense side
+  // batched prepares (blockIdx.y = b < LINE_NB): site arrays and dense-side offset of each
+  const double2* site_in[LINE_NB];  // forward: pair density at the sites
+  double2* site_out[LINE_NB];       // inverse: potential at the sites
+  long long in_ls, out_ls;          // element stride between lines of the dense side
+  long long in_bs, out_bs;          // element stride between batched prepares on the dense side
 };
 
 template <int C, int R1, int R2, int NSUB>
@@ -85,7 +89,7 @@ __global__ __launch_bounds__((line_threads<C, R1, R2, NSUB>())) void k_linepass(
     for (int e = tid; e < C * L; e += NTH) lds[e] = zero;
     __syncthreads();
     for (int k = s0 + tid; k < s1; k += NTH) {
-      const double2 v = a.site_in[k];
+      const double2 v = a.site_in[blockIdx.y][k];
       lds[a.site_ccell[k] - cell0] = mk<TC>((TC)v.x, (TC)v.y);
     }
     __syncthreads();
@@ -112,7 +116,7 @@ __global__ __launch_bounds__((line_threads<C, R1, R2, NSUB>())) void k_linepass(
 #pragma unroll
       for (int m = 0; m < R1; ++m) w[m] = lds[xi(c2, s2, k2, m)];
       rdft<R1, false, TC>(w);
-      C2<TO>* o = out + (size_t)a.lines[b0 + c2] * a.out_ls + k2 * NSUB + s2;
+      C2<TO>* o = out + (size_t)blockIdx.y * a.out_bs + (size_t)a.lines[b0 + c2] * a.out_ls + k2 * NSUB + s2;
 #pragma unroll
       for (int k1 = 0; k1 < R1; ++k1) o[k1 * R2 * NSUB] = cvt<TC, TO>(w[k1]);
     }
@@ -122,7 +126,7 @@ __global__ __launch_bounds__((line_threads<C, R1, R2, NSUB>())) void k_linepass(
     // costs more than their loads)
     if (ok2) {
       CT w[R1];
-      const C2<TI>* ip = in + (size_t)a.lines[b0 + c2] * a.in_ls + k2 * NSUB + s2;
+      const C2<TI>* ip = in + (size_t)blockIdx.y * a.in_bs + (size_t)a.lines[b0 + c2] * a.in_ls + k2 * NSUB + s2;
 #pragma unroll
       for (int k1 = 0; k1 < R1; ++k1) w[k1] = cvt<TI, TC>(ip[k1 * R2 * NSUB]);
       rdft<R1, true, TC>(w);
@@ -161,13 +165,13 @@ __global__ __launch_bounds__((line_threads<C, R1, R2, NSUB>())) void k_linepass(
         r.x += (double)u1.x;
         r.y += (double)u1.y;
       }
-      a.site_out[k] = r;
+      a.site_out[blockIdx.y][k] = r;
     }
   }
 }
 
 struct LineLaunch {
-  unsigned blocks;
+  unsigned blocks, nb;
   hipStream_t stream;
   const void* in;
   void* out;
@@ -182,7 +186,7 @@ hipError_t line_go(const LineLaunch& L) {
   constexpr int nth = line_threads<C, R1, R2, NSUB>();
   constexpr size_t lds = line_lds<TC, R1, R2, NSUB, C>();
   static_assert(lds <= 65536, "line pass tile exceeds 64 KiB of LDS");
-  hipLaunchKernelGGL((k_linepass<TI, TC, TO, MODE, R1, R2, NSUB, C>), dim3(L.blocks), dim3(nth), lds, L.stream,
+  hipLaunchKernelGGL((k_linepass<TI, TC, TO, MODE, R1, R2, NSUB, C>), dim3(L.blocks, L.nb), dim3(nth), lds, L.stream,
                      (const C2<TI>*)L.in, (C2<TO>*)L.out, L.a, (const C2<TC>*)L.T2L);
   return hipGetLastError();
 }

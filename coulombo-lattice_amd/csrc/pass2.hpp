@@ -44,6 +44,9 @@ struct P2Args {
   // rows without sites (column kernel only; the general kernel rejects it)
   const uint32_t* rowmask;
   int mask_os;
+  // batched prepares (column kernel only): count and element strides
+  int nb;
+  long long in_bs, out_bs;
 };
 
 // register DFT of size R: compile-time when RT > 0, else dispatched on R

@@ -193,6 +193,36 @@ def test_site_lines_match_oracle(prec, dims):
 
 
 @pytest.mark.parametrize("prec", [clat.PREC_FP64, clat.PREC_FP32, clat.PREC_FP32X], ids=["fp64", "fp32", "fp32x"])
+def test_batched_prepares(prec, monkeypatch):
+    """Several left products per launch (up to four): 5 states on a 32^3 grid
+    give a cover of 15 left products, not a multiple of the batch; the values
+    match the oracle main loop and are bit-identical to one prepare per launch."""
+    print("case batched", prec, flush=True)
+    w = zb.workload(8, 8, 8, 3, 2, config=0, sigma_frac=0.2, n_orb=4)
+    X, Y, Z = w["dims"]
+    quads = clat.pattern_enumerate("****", 5, 3)
+
+    def run():
+        eng = clat.Engine((X, Y, Z), prec, clat.SCHED_SHARED)
+        eng.initialize(clat.InteractionSimple(w["step"], 0.0, 10.89))
+        eng.set_atoms(w["cells"], 4)
+        for st in w["states"]:
+            eng.add_state(st)
+        return eng.compute_integrals(quads)
+
+    got = run()
+    monkeypatch.setenv("CLAT_BATCH", "1")
+    one = run()
+    assert np.array_equal(got, one)
+    rho = O.products(np.stack(w["states"]))
+    Gh = O.ghat(O.gtable(X, Y, Z, w["step"], 0.0, 10.89))
+    want = O.compute_integrals([tuple(q) for q in quads], 5, rho, w["cells"], (X, Y, Z), Gh)
+    scale = np.max(np.abs(want))
+    err = np.abs(got - want) / np.maximum(np.abs(want), 1e-3 * scale)
+    assert np.max(err) < TOL[prec], np.max(err)
+
+
+@pytest.mark.parametrize("prec", [clat.PREC_FP64, clat.PREC_FP32, clat.PREC_FP32X], ids=["fp64", "fp32", "fp32x"])
 def test_thomas_fermi(prec):
     """InteractionThomasFermi (Interaction.cpp:18-39) table vs the oracle's."""
     print("case thomas_fermi", prec, flush=True)
