@@ -881,17 +881,17 @@ void dct_axis(clat_engine* e, int axis, double scale) {
   e->kbegin();
   if (axis == 0) {
     const int ncols = Y1 * Z1;
-    dim3 grid((ncols + C - 1) / C);
+    dim3 grid((ncols + 2 * C - 1) / (2 * C));
     hipLaunchKernelGGL((k_dct1<16, true>), grid, dim3(NT), lds, e->stream, e->G64, p, e->dtw[0], ncols, C,
                        log2i(C), 0LL, 1LL, (long long)X1, ncols, scale);
   } else if (axis == 1) {
     const int ncols = X1;
-    dim3 grid(((ncols + C - 1) / C) * Z1);
+    dim3 grid(((ncols + 2 * C - 1) / (2 * C)) * Z1);
     hipLaunchKernelGGL((k_dct1<16, false>), grid, dim3(NT), lds, e->stream, e->G64, p, e->dtw[1], ncols, C,
                        log2i(C), (long long)X1 * Y1, (long long)X1, 1LL, ncols, scale);
   } else {
     const int ncols = X1 * Y1;
-    dim3 grid((ncols + C - 1) / C);
+    dim3 grid((ncols + 2 * C - 1) / (2 * C));
     hipLaunchKernelGGL((k_dct1<16, false>), grid, dim3(NT), lds, e->stream, e->G64, p, e->dtw[2], ncols, C,
                        log2i(C), 0LL, (long long)X1 * Y1, 1LL, ncols, scale);
   }

@@ -305,15 +305,18 @@ __global__ __launch_bounds__(NT) void k_dct1(double* __restrict__ G, AxisPlan p,
                                              const double2* __restrict__ tw, int nlines, int C, int lgC,
                                              long long os, long long is, long long cs, int ncols,
                                              double scale) {
+  // Two real columns per complex line: the even extensions of columns c and
+  // c + C are real and even, so their transforms are real and come back as
+  // the real and imaginary parts of one complex FFT.
   extern __shared__ __align__(16) unsigned char smem_raw[];
   double2* buf = reinterpret_cast<double2*>(smem_raw);
   const int L2 = p.L;      // 2M
   const int M = L2 / 2;
-  // block -> (outer o, first column c0)
-  const int ncb = (ncols + C - 1) / C;
+  // block -> (outer o, first column c0); 2C columns per block
+  const int ncb = (ncols + 2 * C - 1) / (2 * C);
   const int o = blockIdx.x / ncb;
-  const int c0 = (blockIdx.x - o * ncb) * C;
-  const int nC = min(C, ncols - c0);
+  const int c0 = (blockIdx.x - o * ncb) * 2 * C;
+  const int nC = min(2 * C, ncols - c0);
   double* base = G + (size_t)o * os + (size_t)c0 * cs;
   const int tot = C * L2;
   for (int e = threadIdx.x; e < tot; e += NT) {
@@ -321,9 +324,10 @@ __global__ __launch_bounds__(NT) void k_dct1(double* __restrict__ G, AxisPlan p,
     if (LM) { c = (int)fdiv((uint32_t)e, p.l_magic); m = e - c * L2; }
     else { c = e & (C - 1); m = e >> lgC; }
     const int i = m <= M ? m : L2 - m;
-    double v = 0.0;
-    if (c < nC) v = base[(size_t)i * is + (size_t)c * cs];
-    buf[LM ? c * L2 + m : m * C + c] = make_double2(v, 0.0);
+    double v0 = 0.0, v1 = 0.0;
+    if (c < nC) v0 = base[(size_t)i * is + (size_t)c * cs];
+    if (c + C < nC) v1 = base[(size_t)i * is + (size_t)(c + C) * cs];
+    buf[LM ? c * L2 + m : m * C + c] = make_double2(v0, v1);
   }
   __syncthreads();
   lds_fft<false, EMAX, LM, double>(buf, tw, p, C, lgC);
@@ -332,7 +336,9 @@ __global__ __launch_bounds__(NT) void k_dct1(double* __restrict__ G, AxisPlan p,
     int c, k;
     if (LM) { c = e / (M + 1); k = e - c * (M + 1); }
     else { c = e & (C - 1); k = e >> lgC; }
-    if (c < nC) base[(size_t)k * is + (size_t)c * cs] = buf[LM ? c * L2 + k : k * C + c].x * scale;
+    const double2 r = buf[LM ? c * L2 + k : k * C + c];
+    if (c < nC) base[(size_t)k * is + (size_t)c * cs] = r.x * scale;
+    if (c + C < nC) base[(size_t)k * is + (size_t)(c + C) * cs] = r.y * scale;
   }
   (void)nlines;
 }
