@@ -975,9 +975,13 @@ double nsum(const double* v, size_t n, size_t stride) {
 void build_products(clat_engine* e) {
   const int P = e->n_states * (e->n_states + 1) / 2;
   if (e->rho_states == e->n_states && e->rho) return;
-  e->dfree(e->rho, e->rho_bytes);
-  e->rho_bytes = (size_t)P * e->n_sites * sizeof(double2);
-  e->rho = (double2*)e->dalloc(e->rho_bytes);
+  // the buffer is kept between runs: a free here would synchronise the device
+  const size_t need = std::max<size_t>(16, (size_t)P * e->n_sites * sizeof(double2));
+  if (e->rho_bytes < need) {
+    e->dfree(e->rho, e->rho_bytes);
+    e->rho = (double2*)e->dalloc(need);
+    e->rho_bytes = need;
+  }
   e->rho_states = e->n_states;
   if (e->n_sites == 0) return;
   // S sites per block (power of two, <= 64): all states of S + 1 sites in about 44 KB of LDS

@@ -137,7 +137,9 @@ struct TextFile {
   }
 };
 
-inline TextFile read_text(const std::string& path) {
+// line starts: 0 and every i + 1 with data[i] == '\n' and i + 1 < size,
+// found in parallel slices with memchr and concatenated in order
+inline TextFile read_text(const std::string& path, int threads = 1) {
   TextFile t;
   FILE* f = std::fopen(path.c_str(), "rb");
   if (!f) failf("could not open file: %s", path.c_str());
@@ -152,9 +154,33 @@ inline TextFile read_text(const std::string& path) {
     }
   }
   std::fclose(f);
-  if (!t.data.empty()) t.starts.push_back(0);
-  for (size_t i = 0; i < t.data.size(); ++i)
-    if (t.data[i] == '\n' && i + 1 < t.data.size()) t.starts.push_back(i + 1);
+  const size_t size = t.data.size();
+  if (size == 0) return t;
+  const int T = std::max(1, std::min<int>(threads, (int)(size / (1 << 20)) + 1));
+  std::vector<std::vector<size_t>> part((size_t)T);
+  std::vector<std::thread> pool;
+  const size_t chunk = (size + T - 1) / T;
+  for (int k = 0; k < T; ++k) {
+    pool.emplace_back([&, k] {
+      const size_t a = std::min(size, (size_t)k * chunk), b = std::min(size, a + chunk);
+      const char* base = t.data.data();
+      const char* p = base + a;
+      const char* e = base + b;
+      while (p < e) {
+        const char* q = static_cast<const char*>(std::memchr(p, '\n', (size_t)(e - p)));
+        if (!q) break;
+        const size_t i = (size_t)(q - base);
+        if (i + 1 < size) part[(size_t)k].push_back(i + 1);
+        p = q + 1;
+      }
+    });
+  }
+  for (auto& th : pool) th.join();
+  size_t total = 1;
+  for (const auto& v : part) total += v.size();
+  t.starts.reserve(total);
+  t.starts.push_back(0);
+  for (const auto& v : part) t.starts.insert(t.starts.end(), v.begin(), v.end());
   return t;
 }
 
@@ -208,7 +234,7 @@ void parallel_for(size_t n, int threads, F&& f) {
 // FunctionCollection::loadAtomsPositions (FunctionCollection.cpp:38-54):
 // lines while three numbers parse; the first other line ends the list.
 inline std::vector<double> read_atoms(const std::string& path, int threads) {
-  TextFile t = read_text(path);
+  TextFile t = read_text(path, threads);
   const size_t nl = t.starts.size();
   std::vector<double> xyz(3 * nl);
   std::vector<char> ok(nl, 0);
@@ -231,7 +257,7 @@ inline std::vector<double> read_atoms(const std::string& path, int threads) {
 // ignored), atom-major / orbital-minor; missing or short lines are an error.
 inline std::vector<double> read_lcao(const std::string& path, int skip, int64_t n_atoms, int n_orb,
                                      int threads) {
-  TextFile t = read_text(path);
+  TextFile t = read_text(path, threads);
   const size_t need = (size_t)n_atoms * n_orb;
   const size_t first = (size_t)skip;
   if (t.starts.size() < first + need) failf("file is truncated: %s", path.c_str());

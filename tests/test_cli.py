@@ -69,6 +69,24 @@ def test_non_lattice_geometry(tmp_path):
     assert rc == 1 and "ERROR: could not find common grid step" in err
 
 
+@pytest.mark.parametrize("trailing_newline", [True, False])
+def test_large_file_line_count(tmp_path, trailing_newline):
+    """Multi-megabyte inputs are split into lines by several threads: the
+    line count must come out exact (one line short is 'truncated', the exact
+    count is accepted), with or without a final newline."""
+    w = zb.workload(16, 16, 8, 0, 1, config=0, sigma_frac=0.2, n_orb=4)
+    zb.write_files(str(tmp_path), w)
+    lines = (tmp_path / "e1.dat").read_text().splitlines()
+    assert len(lines) == 4 * len(w["cells"]) > 60000
+    body = "\n".join(lines[:-1]) + ("\n" if trailing_newline else "")
+    (tmp_path / "e2.dat").write_text(body)
+    rc, _, err = run(["--atoms=zb.3d", "--orbitals=4", "--threads-per-node=8", "e2.dat"], str(tmp_path))
+    assert rc == 1 and "ERROR: file is truncated: e2.dat" in err, err
+    (tmp_path / "e3.dat").write_text("\n".join(lines) + ("\n" if trailing_newline else ""))
+    rc, _, err = run(["--atoms=zb.3d", "--orbitals=4", "--threads-per-node=8", "e3.dat"], str(tmp_path))
+    assert "Reading wavefunctions... (" in err and "truncated" not in err, err
+
+
 def test_stage_messages(small):
     """Timer lines in the reference's format: '<stage>... (x.xxx s)'."""
     d, _ = small
