@@ -89,7 +89,7 @@ __global__ __launch_bounds__((line_threads<C, R1, R2, NSUB>())) void k_linepass(
     for (int e = tid; e < C * L; e += NTH) lds[e] = zero;
     __syncthreads();
     for (int k = s0 + tid; k < s1; k += NTH) {
-      const double2 v = a.site_in[blockIdx.y][k];
+      const double2 v = ldg_stream(a.site_in[blockIdx.y] + k);
       lds[a.site_ccell[k] - cell0] = mk<TC>((TC)v.x, (TC)v.y);
     }
     __syncthreads();
@@ -128,7 +128,7 @@ __global__ __launch_bounds__((line_threads<C, R1, R2, NSUB>())) void k_linepass(
       CT w[R1];
       const C2<TI>* ip = in + (size_t)blockIdx.y * a.in_bs + (size_t)a.lines[b0 + c2] * a.in_ls + k2 * NSUB + s2;
 #pragma unroll
-      for (int k1 = 0; k1 < R1; ++k1) w[k1] = cvt<TI, TC>(ip[k1 * R2 * NSUB]);
+      for (int k1 = 0; k1 < R1; ++k1) w[k1] = cvt<TI, TC>(ldg_stream(ip + k1 * R2 * NSUB));
       rdft<R1, true, TC>(w);
       const uint32_t tstep = (uint32_t)(2 * k2 + s2) * SC;
 #pragma unroll
