@@ -458,6 +458,13 @@ __global__ void k_gather(const C2<T>* __restrict__ V, const int32_t* __restrict_
 // S + 1: conflict-free 16-byte accesses), G = blockDim / S thread groups
 // share the products of each site.
 // ---------------------------------------------------------------------------
+// read-once double2 load with the nontemporal hint
+__device__ __forceinline__ double2 nt_load(const double2* p) {
+  typedef double d2v __attribute__((ext_vector_type(2)));
+  const d2v u = __builtin_nontemporal_load(reinterpret_cast<const d2v*>(p));
+  return make_double2(u.x, u.y);
+}
+
 __device__ __forceinline__ void pair_of(int p, int& L, int& R) {
   L = (int)((sqrt(8.0 * p + 1.0) - 1.0) * 0.5);
   while (L * (L + 1) / 2 > p) --L;
@@ -483,11 +490,11 @@ __global__ __launch_bounds__(NT) void k_products(const double2* __restrict__ coe
     for (; s + 4 <= n_states; s += 4) {
       double2 v[4];
 #pragma unroll
-      for (int j = 0; j < 4; ++j) v[j] = src[(size_t)(s + j) * n_sites * n_orb];
+      for (int j = 0; j < 4; ++j) v[j] = nt_load(src + (size_t)(s + j) * n_sites * n_orb);
 #pragma unroll
       for (int j = 0; j < 4; ++j) dst[(size_t)(s + j) * n_orb * SP] = v[j];
     }
-    for (; s < n_states; ++s) dst[(size_t)s * n_orb * SP] = src[(size_t)s * n_sites * n_orb];
+    for (; s < n_states; ++s) dst[(size_t)s * n_orb * SP] = nt_load(src + (size_t)s * n_sites * n_orb);
   }
   __syncthreads();
   const int a = threadIdx.x % S, g = threadIdx.x / S, G = NT / S;
@@ -580,8 +587,8 @@ __global__ __launch_bounds__(TX * TX) void k_overlap_partial(const double2* __re
       const int row = e / OV_S, col = e - row * OV_S;
       const int64_t site = sb + col;
       const bool ok = e < T * OV_S && site < s1;
-      bv[j] = (ok && l0 + row < n_left) ? Vat[(size_t)(l0 + row) * n_sites + site] : make_double2(0.0, 0.0);
-      br[j] = (ok && r0 + row < n_right) ? rho[(size_t)(r0 + row) * n_sites + site] : make_double2(0.0, 0.0);
+      bv[j] = (ok && l0 + row < n_left) ? nt_load(Vat + (size_t)(l0 + row) * n_sites + site) : make_double2(0.0, 0.0);
+      br[j] = (ok && r0 + row < n_right) ? nt_load(rho + (size_t)(r0 + row) * n_sites + site) : make_double2(0.0, 0.0);
     }
 #pragma unroll
     for (int j = 0; j < NE; ++j) {
