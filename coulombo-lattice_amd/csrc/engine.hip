@@ -7,6 +7,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <array>
 #include <string>
 #include <vector>
 
@@ -263,6 +264,9 @@ struct clat_engine {
   int nb_max = 1, cur_nb = 1;
   const double2* b_in[LINE_NB] = {};
   double2* b_out[LINE_NB] = {};
+  const double2* b_in2[LINE_NB] = {};  // second real density packed into the prepare (or null)
+  double2* b_out2[LINE_NB] = {};
+  bool pack = true;                     // pairs of real (diagonal) densities share a prepare
   size_t a_set = 0, b_set = 0, d_set = 0;  // elements of A, B, D per prepare
   bool xline = false;                // x passes on the line kernel, y passes skip empty lines
   int n_states = 0, cap_states = 0;
@@ -501,6 +505,8 @@ void launch_line(clat_engine* e, const void* in, void* out, long long bs, int ki
   for (int b = 0; b < LINE_NB; ++b) {
     a.site_in[b] = MODE == PM_FWD ? e->b_in[b] : nullptr;
     a.site_out[b] = MODE == PM_INV ? e->b_out[b] : nullptr;
+    a.site_in2[b] = MODE == PM_FWD && b < e->cur_nb ? e->b_in2[b] : nullptr;
+    a.site_out2[b] = MODE == PM_INV && b < e->cur_nb ? e->b_out2[b] : nullptr;
   }
   a.in_ls = a.out_ls = 2LL * e->Xi;
   a.in_bs = a.out_bs = bs;
@@ -1250,6 +1256,7 @@ int clat_create(int X, int Y, int Z, int precision, int schedule, clat_engine** 
     if (e->xline && e->conv_tiled)
       while (e->nb_max < LINE_NB && (size_t)(2 * e->nb_max) * set_bytes <= ((size_t)16 << 30)) e->nb_max *= 2;
     if (const char* g = std::getenv("CLAT_BATCH")) e->nb_max = std::max(1, std::min(e->nb_max, std::atoi(g)));
+    if (const char* g = std::getenv("CLAT_PACK")) e->pack = std::atoi(g) != 0;
     e->a_bytes = e->nb_max * e->a_set * e->s_a;
     e->A = e->dalloc(e->a_bytes);
     if (e->shared()) {
@@ -1556,16 +1563,41 @@ int clat_compute_integrals(clat_engine* e, const int32_t* quads, int64_t n, int 
       e->Vat = (double2*)e->dalloc(vat_need * sizeof(double2));
       e->vat_cap = vat_need;
     }
-    for (int sl = 0; sl < nL && e->n_sites > 0;) {
-      const int nb = std::min(e->nb_max, nL - sl);
+    // prepare units: on the line-kernel path two real (diagonal) densities
+    // share one complex prepare, their potentials being its real and
+    // imaginary parts (the kernel is real)
+    std::vector<std::array<int, 2>> units;
+    {
+      std::vector<int> reals;
+      const bool pk = e->pack && e->xline && e->shared();
+      for (int sl = 0; sl < nL; ++sl) {
+        int L = 0;
+        while ((L + 1) * (L + 2) / 2 <= slots[(size_t)sl]) ++L;
+        const bool diag = slots[(size_t)sl] == L * (L + 1) / 2 + L;
+        if (pk && diag) reals.push_back(sl);
+        else units.push_back({sl, -1});
+      }
+      for (size_t i = 0; i < reals.size(); i += 2)
+        units.push_back({reals[i], i + 1 < reals.size() ? reals[i + 1] : -1});
+    }
+    const int nU = (int)units.size();
+    for (int u = 0; u < nU && e->n_sites > 0;) {
+      const int nb = std::min(e->nb_max, nU - u);
       e->cur_nb = nb;
       for (int b = 0; b < nb; ++b) {
-        e->b_in[b] = e->rho + (size_t)slots[(size_t)(sl + b)] * e->n_sites;
-        e->b_out[b] = e->Vat + (size_t)(sl + b) * e->n_sites;
+        const int s0 = units[(size_t)(u + b)][0], s1 = units[(size_t)(u + b)][1];
+        e->b_in[b] = e->rho + (size_t)slots[(size_t)s0] * e->n_sites;
+        e->b_out[b] = e->Vat + (size_t)s0 * e->n_sites;
+        e->b_in2[b] = s1 >= 0 ? e->rho + (size_t)slots[(size_t)s1] * e->n_sites : nullptr;
+        e->b_out2[b] = s1 >= 0 ? e->Vat + (size_t)s1 * e->n_sites : nullptr;
       }
       prepare_sites(e, e->b_in[0], e->b_out[0]);
+      for (int b = 0; b < LINE_NB; ++b) {
+        e->b_in2[b] = nullptr;
+        e->b_out2[b] = nullptr;
+      }
       e->cur_nb = 1;
-      sl += nb;
+      u += nb;
     }
     HIPX(hipEventRecord(e->stage_ev[3], e->stream));
     // overlaps of every (slot, product) pair

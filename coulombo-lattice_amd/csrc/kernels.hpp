@@ -511,7 +511,9 @@ __global__ __launch_bounds__(NT) void k_products(const double2* __restrict__ coe
       re += x.x * y.x + x.y * y.y;
       im += x.x * y.y - x.y * y.x;
     }
-    rho[(size_t)p * n_sites + k0 + a] = make_double2(re, im);
+    // a diagonal density |c_L|^2 is real: its imaginary part is zero, not
+    // the rounding residue of the contracted products
+    rho[(size_t)p * n_sites + k0 + a] = make_double2(re, L == R ? 0.0 : im);
   }
 }
 
@@ -543,7 +545,7 @@ __global__ void k_products_direct(const double2* __restrict__ coeffs, int64_t n_
       re += x.x * y.x + x.y * y.y;
       im += x.x * y.y - x.y * y.x;
     }
-    rho[t] = make_double2(re, im);
+    rho[t] = make_double2(re, fL == fR ? 0.0 : im);
   }
 }
 

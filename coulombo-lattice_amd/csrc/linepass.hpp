@@ -27,6 +27,11 @@ struct LineArgs {
   // batched prepares (blockIdx.y = b < LINE_NB): site arrays and dense-side offset of each
   const double2* site_in[LINE_NB];  // forward: pair density at the sites
   double2* site_out[LINE_NB];       // inverse: potential at the sites
+  // two real densities in one prepare (null when unused): the line carries
+  // site_in[b].x + i site_in2[b].x, and since the kernel is real the potential
+  // splits back into site_out[b] = (Re, 0) and site_out2[b] = (Im, 0)
+  const double2* site_in2[LINE_NB];
+  double2* site_out2[LINE_NB];
   long long in_ls, out_ls;          // element stride between lines of the dense side
   long long in_bs, out_bs;          // element stride between batched prepares on the dense side
 };
@@ -88,8 +93,10 @@ __global__ __launch_bounds__((line_threads<C, R1, R2, NSUB>())) void k_linepass(
     // ---------------------------------------------------------- scatter
     for (int e = tid; e < C * L; e += NTH) lds[e] = zero;
     __syncthreads();
+    const double2* in2 = a.site_in2[blockIdx.y];
     for (int k = s0 + tid; k < s1; k += NTH) {
-      const double2 v = ldg_stream(a.site_in[blockIdx.y] + k);
+      double2 v = ldg_stream(a.site_in[blockIdx.y] + k);
+      if (in2) v.y = ldg_stream(in2 + k).x;
       lds[a.site_ccell[k] - cell0] = mk<TC>((TC)v.x, (TC)v.y);
     }
     __syncthreads();
@@ -155,6 +162,7 @@ __global__ __launch_bounds__((line_threads<C, R1, R2, NSUB>())) void k_linepass(
     }
     __syncthreads();
     // ---------------------------------------------------------- gather
+    double2* out2 = a.site_out2[blockIdx.y];
     for (int k = s0 + tid; k < s1; k += NTH) {
       const int cc = a.site_ccell[k] - cell0;
       const int c = cc / L, x = cc - c * L;
@@ -165,7 +173,12 @@ __global__ __launch_bounds__((line_threads<C, R1, R2, NSUB>())) void k_linepass(
         r.x += (double)u1.x;
         r.y += (double)u1.y;
       }
-      a.site_out[blockIdx.y][k] = r;
+      if (out2) {
+        a.site_out[blockIdx.y][k] = make_double2(r.x, 0.0);
+        out2[k] = make_double2(r.y, 0.0);
+      } else {
+        a.site_out[blockIdx.y][k] = r;
+      }
     }
   }
 }

@@ -90,6 +90,11 @@ def test_column_kernels_match_oracle(prec, sched, dims):
     assert err < TOL[prec] * 0.1, err
 
 
+def _rel(a, b):
+    scale = np.max(np.abs(b))
+    return np.max(np.abs(a - b) / np.maximum(np.abs(b), 1e-3 * scale))
+
+
 def _example():
     d = os.path.join(os.path.dirname(__file__), "golden", "example")
     xyz = np.loadtxt(os.path.join(d, "X.3d"), usecols=(0, 1, 2))
@@ -214,6 +219,11 @@ def test_batched_prepares(prec, monkeypatch):
     monkeypatch.setenv("CLAT_BATCH", "1")
     one = run()
     assert np.array_equal(got, one)
+    # the five diagonal densities go two to a prepare (real and imaginary
+    # part); one prepare each gives the same values to rounding
+    monkeypatch.setenv("CLAT_PACK", "0")
+    unpacked = run()
+    assert _rel(got, unpacked) < (1e-12 if prec == clat.PREC_FP64 else 2e-6), _rel(got, unpacked)
     rho = O.products(np.stack(w["states"]))
     Gh = O.ghat(O.gtable(X, Y, Z, w["step"], 0.0, 10.89))
     want = O.compute_integrals([tuple(q) for q in quads], 5, rho, w["cells"], (X, Y, Z), Gh)
