@@ -278,6 +278,36 @@ inline std::vector<double> read_lcao(const std::string& path, int skip, int64_t 
   return c;
 }
 
+// All LCAO files of a run, several at a time: up to `threads` files are read
+// concurrently, the threads split between them.  Errors are reported for the
+// first failing file in argument order, as a sequential read would.
+inline std::vector<std::vector<double>> read_lcao_all(const std::vector<std::string>& paths, int skip,
+                                                      int64_t n_atoms, int n_orb, int threads) {
+  const int nf = (int)paths.size();
+  std::vector<std::vector<double>> out((size_t)nf);
+  if (nf == 0) return out;
+  const int workers = std::max(1, std::min(threads, nf));
+  const int per = std::max(1, threads / workers);
+  std::vector<std::string> err((size_t)nf);
+  std::atomic<int> next{0};
+  std::vector<std::thread> pool;
+  for (int w = 0; w < workers; ++w) {
+    pool.emplace_back([&] {
+      for (int i = next.fetch_add(1); i < nf; i = next.fetch_add(1)) {
+        try {
+          out[(size_t)i] = read_lcao(paths[(size_t)i], skip, n_atoms, n_orb, per);
+        } catch (const std::exception& ex) {
+          err[(size_t)i] = ex.what();
+        }
+      }
+    });
+  }
+  for (auto& th : pool) th.join();
+  for (int i = 0; i < nf; ++i)
+    if (!err[(size_t)i].empty()) throw Fail(err[(size_t)i]);
+  return out;
+}
+
 inline std::string basename_of(const std::string& path) {
   auto s = path.find_last_of('/');
   return s == std::string::npos ? path : path.substr(s + 1);

@@ -92,20 +92,26 @@ void run(const cli::Options& o) {
   const auto& args = o.args();
   const int n_inputs = (int)args.size();
   if (n_inputs >= std::numeric_limits<short>::max()) cli::failf("too many input files");
+  // name checks in argument order (coulombo.cpp:96-117); the files before the
+  // first bad name are read first, so that their own errors come first
   int n_h = 0, n_e = 0;
-  std::vector<std::vector<double>> states;
-  for (const std::string& a : args) {
-    const std::string base = cli::basename_of(a);
+  std::string name_error;
+  size_t n_ok = args.size();
+  for (size_t i = 0; i < args.size() && name_error.empty(); ++i) {
+    const std::string base = cli::basename_of(args[i]);
     if (!base.empty() && base[0] == 'h') {
-      if (n_e) cli::failf("hole states must appear before electron states");
-      ++n_h;
+      if (n_e) name_error = "hole states must appear before electron states";
+      else ++n_h;
     } else if (!base.empty() && base[0] == 'e') {
       ++n_e;
     } else {
-      cli::failf("invalid state file name: %s", a.c_str());
+      name_error = "invalid state file name: " + args[i];
     }
-    states.push_back(cli::read_lcao(a, skip, n_atoms, n_orb, threads));
+    if (!name_error.empty()) n_ok = i;
   }
+  std::vector<std::vector<double>> states = cli::read_lcao_all(
+      std::vector<std::string>(args.begin(), args.begin() + (long)n_ok), skip, n_atoms, n_orb, threads);
+  if (!name_error.empty()) cli::failf("%s", name_error.c_str());
 
   timer.start("Preparing plan");
   std::vector<int32_t> quads;

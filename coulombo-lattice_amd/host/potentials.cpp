@@ -73,12 +73,9 @@ void run(const cli::Options& o) {
   timer.start("Reading wavefunctions");
   const auto& args = o.args();
   if ((int)args.size() >= std::numeric_limits<short>::max()) cli::failf("too many input files");
-  std::vector<std::vector<double>> states;
+  std::vector<std::vector<double>> states = cli::read_lcao_all(args, skip, n_atoms, n_orb, threads);
   std::vector<std::string> names;
-  for (const std::string& a : args) {
-    states.push_back(cli::read_lcao(a, skip, n_atoms, n_orb, threads));
-    names.push_back(cli::basename_of(a));
-  }
+  for (const std::string& a : args) names.push_back(cli::basename_of(a));
 
   timer.start("Initializing calculator");
   double onsite = 0.0;

@@ -49,6 +49,10 @@ def test_usage_without_inputs(tmp_path):
     (["--atoms=zb.3d", "--orbitals=4", "e1.dat", "h1.dat"], "ERROR: hole states must appear before electron states"),
     (["--atoms=zb.3d", "--orbitals=4", "x1.dat"], "ERROR: invalid state file name: x1.dat"),
     (["--atoms=zb.3d", "--orbitals=5", "e1.dat"], "ERROR: file is truncated: e1.dat"),
+    # a file before a bad name fails first, as in the reference's sequential read
+    (["--atoms=zb.3d", "--orbitals=5", "h1.dat", "x1.dat"], "ERROR: file is truncated: h1.dat"),
+    (["--atoms=zb.3d", "--orbitals=4", "h1.dat", "e1.dat", "h2.dat"],
+     "ERROR: hole states must appear before electron states"),
     (["--atoms=zb.3d", "--orbitals=4", "--integrals=e0ee", "e1.dat"], "ERROR: invalid character '0' in pattern"),
     (["--atoms=zb.3d", "--orbitals=4", "--integrals=eeee,", "e1.dat"], "ERROR: invalid empty pattern"),
 ])
