@@ -98,6 +98,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--schedule", choices=["shared", "rounds"], default="shared")
+    ap.add_argument("--integrals", default="eeee,hhhh,ehhe,eheh",
+                    help="integral pattern list (ijkl with --cells 32 32 32 --states 16 16: the batched form)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a plain device-to-device copy of MIB MiB (counter calibration)")
     args = ap.parse_args()
@@ -125,7 +127,7 @@ def main():
     t0 = time.time()
     w = zb.workload(nx, ny, nz, nh, ne, config=1, sigma_frac=0.15)
     X, Y, Z = w["dims"]
-    quads = clat.pattern_enumerate("eeee,hhhh,ehhe,eheh", nh + ne, nh)
+    quads = clat.pattern_enumerate(args.integrals, nh + ne, nh)
     log("workload: %d atoms, grid %dx%dx%d, %d states, %d integrals (%.1f s)"
         % (len(w["cells"]), X, Y, Z, nh + ne, len(quads), time.time() - t0))
     sched = clat.SCHED_SHARED if args.schedule == "shared" else clat.SCHED_ROUNDS
@@ -190,8 +192,8 @@ def main():
         "ms_per_step": ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "c64 storage, f64 arithmetic and accumulation",
         "data": "synthetic (seeded zincblende, see zincblende.py)",
-        "config": {"workload": "zincblende %dx%dx%d cells = %d atoms, grid %dx%dx%d, %d states, "
-                               "eeee,hhhh,ehhe,eheh" % (nx, ny, nz, len(w["cells"]), X, Y, Z, nh + ne),
+        "config": {"workload": "zincblende %dx%dx%d cells = %d atoms, grid %dx%dx%d, %d states, %s"
+                               % (nx, ny, nz, len(w["cells"]), X, Y, Z, nh + ne, args.integrals),
                    "integrals": len(quads), "schedule": args.schedule},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
                      "frac": achieved / PEAK_HBM_GBS, "traffic": traffic_of(dom), "avg_launch_ms": avg_ms,

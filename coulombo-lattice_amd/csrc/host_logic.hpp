@@ -217,7 +217,8 @@ inline std::vector<int32_t> enumerate_integrals(const Pattern& pat, int n) {
 // ---------------------------------------------------------------------------
 // Greedy vertex cover (src/Graph.hpp:49-91).
 // ---------------------------------------------------------------------------
-inline std::vector<int> vertex_cover(int nv, const std::set<std::pair<int, int>>& edges) {
+// edges: sorted and unique (the iteration order of the reference's std::set)
+inline std::vector<int> vertex_cover(int nv, const std::vector<std::pair<int, int>>& edges) {
   std::set<int> cover;
   for (const auto& e : edges)
     if (e.first == e.second) cover.insert(e.first);
@@ -250,6 +251,22 @@ inline std::vector<int> vertex_cover(int nv, const std::set<std::pair<int, int>>
   return std::vector<int>(cover.begin(), cover.end());
 }
 
+inline std::vector<int> vertex_cover(int nv, const std::set<std::pair<int, int>>& edges) {
+  return vertex_cover(nv, std::vector<std::pair<int, int>>(edges.begin(), edges.end()));
+}
+
+// LSD radix sort of 64-bit keys, 16 bits per pass over the bits in use
+inline void radix_sort_u64(std::vector<uint64_t>& v, int bits) {
+  std::vector<uint64_t> tmp(v.size());
+  for (int shift = 0; shift < bits; shift += 16) {
+    std::vector<size_t> count(65537, 0);
+    for (uint64_t x : v) ++count[((x >> shift) & 0xFFFF) + 1];
+    for (int i = 0; i < 65536; ++i) count[i + 1] += count[i];
+    for (uint64_t x : v) tmp[count[(x >> shift) & 0xFFFF]++] = x;
+    v.swap(tmp);
+  }
+}
+
 // ---------------------------------------------------------------------------
 // MasterPlanner (src/Planner.hpp:65-145).
 // ---------------------------------------------------------------------------
@@ -275,26 +292,57 @@ inline PlanProduct plan_product(int iL, int iR) {
 inline std::vector<PlanStep> make_plan(const int32_t* quads, int64_t n, int n_states) {
   if (n > std::numeric_limits<int>::max()) throw Error(1, "too many integrals to compute");
   std::vector<PlanStep> steps((size_t)n);
-  const int P = n_states * (n_states + 1) / 2;
-  std::set<std::pair<int, int>> edges;
+  const int64_t P = (int64_t)n_states * (n_states + 1) / 2;
   for (int64_t k = 0; k < n; ++k) {
     const int32_t* q = quads + 4 * k;
     for (int j = 0; j < 4; ++j)
       if (q[j] < 1 || q[j] > n_states) throw Error(1, "integral state index out of range");
     steps[(size_t)k] = PlanStep{(int)k, plan_product(q[0], q[3]), plan_product(q[1], q[2])};
-    edges.insert({steps[(size_t)k].left.product, steps[(size_t)k].right.product});
   }
-  const std::vector<int> cv = vertex_cover(P, edges);
+  // the product graph's edges, sorted and unique: a bitmap over P x P when
+  // it is small, else a sort of the edge keys
+  std::vector<std::pair<int, int>> edges;
+  if (P * P <= ((int64_t)1 << 28)) {
+    std::vector<uint64_t> bits((size_t)((P * P + 63) / 64), 0);
+    for (const PlanStep& s : steps) {
+      const uint64_t e = (uint64_t)s.left.product * P + s.right.product;
+      bits[e >> 6] |= (uint64_t)1 << (e & 63);
+    }
+    for (size_t w = 0; w < bits.size(); ++w)
+      for (uint64_t b = bits[w]; b; b &= b - 1) {
+        const uint64_t e = w * 64 + (uint64_t)__builtin_ctzll(b);
+        edges.push_back({(int)(e / P), (int)(e % P)});
+      }
+  } else {
+    std::vector<uint64_t> ek;
+    ek.reserve(steps.size());
+    for (const PlanStep& s : steps) ek.push_back((uint64_t)s.left.product * P + s.right.product);
+    std::sort(ek.begin(), ek.end());
+    ek.erase(std::unique(ek.begin(), ek.end()), ek.end());
+    for (uint64_t e : ek) edges.push_back({(int)(e / P), (int)(e % P)});
+  }
+  const std::vector<int> cv = vertex_cover((int)P, edges);
   std::vector<char> in_cover((size_t)P, 0);
   for (int v : cv) in_cover[(size_t)v] = 1;
   for (PlanStep& s : steps)
     if (!in_cover[(size_t)s.left.product]) std::swap(s.left, s.right);
-  std::sort(steps.begin(), steps.end(), [](const PlanStep& a, const PlanStep& b) {
-    if (a.left.product < b.left.product) return true;
-    if (a.left.product > b.left.product) return false;
-    if (a.right.product < b.right.product) return true;
-    if (a.right.product > b.right.product) return false;
-    return (a.left.conj == a.right.conj) && (b.left.conj != b.right.conj);
+  // order (Planner.hpp:94-101): left product, right product, then the steps
+  // whose two products have equal conj flags before the others
+  auto rc = [](const PlanStep& s) { return (s.left.conj != s.right.conj) ? 1 : 0; };
+  if (P * P * 2 < ((int64_t)1 << 32)) {
+    std::vector<uint64_t> key;
+    key.reserve(steps.size());
+    for (const PlanStep& s : steps)
+      key.push_back((((uint64_t)s.left.product * P + s.right.product) * 2 + rc(s)) << 31 | (uint64_t)s.id);
+    radix_sort_u64(key, 64);
+    std::vector<PlanStep> sorted(steps.size());
+    for (size_t i = 0; i < key.size(); ++i) sorted[i] = steps[(size_t)(key[i] & 0x7FFFFFFF)];
+    return sorted;
+  }
+  std::sort(steps.begin(), steps.end(), [&rc](const PlanStep& a, const PlanStep& b) {
+    if (a.left.product != b.left.product) return a.left.product < b.left.product;
+    if (a.right.product != b.right.product) return a.right.product < b.right.product;
+    return rc(a) < rc(b);
   });
   return steps;
 }
