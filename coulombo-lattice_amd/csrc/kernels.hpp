@@ -581,8 +581,10 @@ __global__ __launch_bounds__(TX * TX) void k_overlap_partial(const double2* __re
 #pragma unroll
       for (int q = 0; q < 4; ++q) acc[i][j][q] = 0.0;
   constexpr int NE = (T * OV_S + NTH - 1) / NTH;  // staged elements per thread and table
-  for (int64_t sb = s0; sb < s1; sb += OV_S) {
-    double2 bv[NE], br[NE];
+  // software pipeline: the next batch of sites is loaded into registers while
+  // the current one is summed from LDS
+  double2 bv[NE], br[NE];
+  auto fetch = [&](int64_t sb) {
 #pragma unroll
     for (int j = 0; j < NE; ++j) {
       const int e = threadIdx.x + j * NTH;
@@ -592,6 +594,9 @@ __global__ __launch_bounds__(TX * TX) void k_overlap_partial(const double2* __re
       bv[j] = (ok && l0 + row < n_left) ? nt_load(Vat + (size_t)(l0 + row) * n_sites + site) : make_double2(0.0, 0.0);
       br[j] = (ok && r0 + row < n_right) ? nt_load(rho + (size_t)(r0 + row) * n_sites + site) : make_double2(0.0, 0.0);
     }
+  };
+  if (s0 < s1) fetch(s0);
+  for (int64_t sb = s0; sb < s1; sb += OV_S) {
 #pragma unroll
     for (int j = 0; j < NE; ++j) {
       const int e = threadIdx.x + j * NTH;
@@ -602,6 +607,7 @@ __global__ __launch_bounds__(TX * TX) void k_overlap_partial(const double2* __re
       }
     }
     __syncthreads();
+    if (sb + OV_S < s1) fetch(sb + OV_S);
 #pragma unroll 4
     for (int s = 0; s < OV_S; ++s) {
       double2 v[3], r[3];
