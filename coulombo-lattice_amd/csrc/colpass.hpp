@@ -142,8 +142,7 @@ __global__ __launch_bounds__((col_threads<C, R1>())) void k_colpass(const C2<TI>
   __shared__ CT twl[2 * L];
   CT twv[NTW];
 #pragma unroll
-  for (int j = 0; j < NTW; ++j)
-    if (tid + j * NTH < 2 * L) twv[j] = T2L[tid + j * NTH];
+  for (int j = 0; j < NTW; ++j) twv[j] = T2L[min(tid + j * NTH, 2 * L - 1)];  // unconditional: stays in registers
 
   // ------------------------------------------------------------------ load
   // FWD / CONV: thread (c, n1 = r) loads x[n1 + R1 n2], n2 < R2
