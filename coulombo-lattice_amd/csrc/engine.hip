@@ -267,6 +267,7 @@ struct clat_engine {
   const double2* b_in2[LINE_NB] = {};  // second real density packed into the prepare (or null)
   double2* b_out2[LINE_NB] = {};
   bool pack = true;                     // pairs of real (diagonal) densities share a prepare
+  bool table_pending = false;           // stage_ms[0] of the last table build not read yet
   size_t a_set = 0, b_set = 0, d_set = 0;  // elements of A, B, D per prepare
   bool xline = false;                // x passes on the line kernel, y passes skip empty lines
   int n_states = 0, cap_states = 0;
@@ -332,6 +333,7 @@ struct clat_engine {
   void kcollect() {
     if (!ktiming) return;
     HIPX(hipStreamSynchronize(stream));
+    // (the table-build events of settle_table are complete from here on)
     for (size_t i = 0; i < kpending.size(); ++i) {
       float ms = 0.f;
       HIPX(hipEventElapsedTime(&ms, kev[2 * i], kev[2 * i + 1]));
@@ -948,11 +950,18 @@ void build_table(clat_engine* e, GArgs g) {
   LAUNCH_CHECK();
   finish_interaction(e);
   HIPX(hipEventRecord(e->stage_ev[5], e->stream));
-  HIPX(hipStreamSynchronize(e->stream));
+  // no host wait here: the next call that waits for the stream reads the time
+  e->table_pending = true;
+}
+
+// time of the last table build, once the stream has passed it
+void settle_table(clat_engine* e) {
+  if (!e->table_pending) return;
   float ms = 0.f;
+  HIPX(hipEventSynchronize(e->stage_ev[5]));
   HIPX(hipEventElapsedTime(&ms, e->stage_ev[0], e->stage_ev[5]));
   e->stage_ms[0] = ms;
-  e->kcollect();
+  e->table_pending = false;
 }
 
 void ensure_dpart(clat_engine* e, size_t n) {
@@ -1699,8 +1708,10 @@ int clat_compute_potential_at_atoms(clat_engine* e, int state, double* v) {
 
 int clat_last_stage_ms(clat_engine* e, double ms[5]) {
   if (!e || !ms) return set_err(CLAT_EINVAL, "invalid arguments");
-  for (int i = 0; i < 5; ++i) ms[i] = e->stage_ms[i];
-  return CLAT_OK;
+  return guarded([&] {
+    settle_table(e);
+    for (int i = 0; i < 5; ++i) ms[i] = e->stage_ms[i];
+  });
 }
 
 int clat_set_kernel_timing(clat_engine* e, int enable) {
