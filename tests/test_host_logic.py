@@ -105,6 +105,20 @@ def test_plan_vs_restatement(spec, n, nh):
     assert [key(*r[1:]) for r in got] == [key(L, lc, R, rc) for _, (L, lc), (R, rc) in want]
 
 
+@pytest.mark.parametrize("n", [40, 320])
+def test_plan_large_state_counts(n):
+    """Many states: the product graph of 40 states goes through the edge
+    bitmap and the radix-sorted steps; 320 states (P^2 past both limits)
+    through the sorted edge list and the comparison sort."""
+    rng = np.random.default_rng(n)
+    quads = rng.integers(1, n + 1, size=(3000, 4)).astype(np.int32)
+    got = clat.plan(quads, n)
+    want = O.plan([tuple(int(v) for v in q) for q in quads], n)
+    assert _plan_key(got) == _plan_key([(k, L, lc, R, rc) for k, (L, lc), (R, rc) in want])
+    key = lambda L, lc, R, rc: (L, R, 0 if lc == rc else 1)  # noqa: E731
+    assert [key(*r[1:]) for r in got] == [key(L, lc, R, rc) for _, (L, lc), (R, rc) in want]
+
+
 def test_plan_product_indexing():
     """Planner.hpp:134-144: product(iL, iR) = iL(iL-1)/2 + iR - 1 with swap+conj."""
     rows = clat.plan([(1, 2, 3, 4)], 4)
