@@ -20,11 +20,13 @@ namespace clat {
 
 struct ColArgs2 {
   int W;                          // columns in the plane
-  // outer index o = blockIdx.y (after the conv reordering) -> element offset
+  // outer index o (after the conv reordering) -> element offset
   //   (o / od) * os_hi + (o % od) * os_lo
   int od;
   long long in_os_hi, in_os_lo, out_os_hi, out_os_lo;
   long long in_bs, out_bs;        // element stride between batched prepares (blockIdx.z)
+  long long in_cbs, out_cbs;      // element stride between blocks of C columns (blockIdx.x): C for
+                                  // row-major planes, C x rows for planes tiled in C-column strips
   // rows: n-domain row n1 + R1 n2 -> byte offset n2 * is_hi + n1 * is_lo;
   // k-domain rows j (y passes only) -> j * is_lo
   unsigned in_is_lo, in_is_hi, out_is_lo, out_is_hi;
@@ -37,6 +39,10 @@ struct ColArgs2 {
   // or written (inverse)
   const uint32_t* rowmask;
   int mask_os;
+  // launch order: 0 = blockIdx.x is the column block and blockIdx.y the outer
+  // index, 1 = the reverse (outer index fastest, so the blocks in flight share
+  // column blocks)
+  int outer_x;
 };
 
 template <typename T>
@@ -117,22 +123,23 @@ __global__ __launch_bounds__((col_threads<C, R1>())) void k_colpass(const C2<TI>
   const int tid = threadIdx.x;
   const int c = tid & (C - 1);
   const int r = tid / C;
-  const int col = blockIdx.x * C + c;
+  const int cbk = a.outer_x ? (int)blockIdx.y : (int)blockIdx.x;  // column block
+  const int col = cbk * C + c;
   const bool okc = col < a.W;
   const CT zero = mk<TC>((TC)0, (TC)0);
   // outer index; the conv pass pairs fy with its reflection 2Y - fy
-  int o = blockIdx.y;
+  int o = a.outer_x ? (int)blockIdx.x : (int)blockIdx.y;
   if (MODE == PM_CONV && a.pair) {
-    const int j = o, last = (int)gridDim.y - 1;
+    const int j = o, last = (int)(a.outer_x ? gridDim.x : gridDim.y) - 1;
     const int t = (j + 1) >> 1;
     o = (j == 0 || j == last) ? (j == 0 ? 0 : a.Y2 >> 1) : ((j & 1) ? t : a.Y2 - t);
   }
   const int oh = o / a.od, ol = o - oh * a.od;
   // per-block bases; every later offset is 32-bit
   const C2<TI>* bin = in + (size_t)blockIdx.z * a.in_bs + (size_t)oh * a.in_os_hi + (size_t)ol * a.in_os_lo +
-                      (size_t)blockIdx.x * C;
+                      (size_t)cbk * a.in_cbs;
   C2<TO>* bout = out + (size_t)blockIdx.z * a.out_bs + (size_t)oh * a.out_os_hi + (size_t)ol * a.out_os_lo +
-                 (size_t)blockIdx.x * C;
+                 (size_t)cbk * a.out_cbs;
   const uint32_t cin = (uint32_t)c * SI, cout = (uint32_t)c * SO;
 
   // pass twiddles W_2L^m, m < 2L, go to LDS: read there by every phase, they
@@ -189,7 +196,7 @@ __global__ __launch_bounds__((col_threads<C, R1>())) void k_colpass(const C2<TI>
   if constexpr (MODE == PM_CONV) {
     const int fy = o * a.gym + a.gya;
     const int ry = min(fy, a.Y2 - fy);
-    Gb = G + ((size_t)(ry * a.npar + a.par) * a.ncb + blockIdx.x) * a.GZ1 * C + c;
+    Gb = G + ((size_t)(ry * a.npar + a.par) * a.ncb + cbk) * a.GZ1 * C + c;
   }
 
 #pragma unroll

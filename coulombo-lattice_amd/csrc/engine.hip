@@ -230,6 +230,7 @@ struct clat_engine {
   float2* dtw32[3] = {};        // same, FP32 (column passes in complex64)
   bool colk = true;             // compile-time column kernels for the y and z passes
   bool conv_tiled = false;      // conv pass on the column kernel (tiled kernel table)
+  bool btile = false;           // B in strips of C columns x Z rows (shared schedule)
   int gc_c = 16, gc_ncb = 1, gc_npar = 1, gc_w = 0;  // kernel-table tiling
   double* G64 = nullptr;  // Ghat octant (Z+1)(Y+1)(X+1), FP64
   float* G32 = nullptr;   // same, FP32 (generic complex64 kernels)
@@ -409,7 +410,9 @@ bool launch_col(clat_engine* e, const PassCfg& pc, const P2Args& a, int n_outer,
   }
   const int C = col_c<TC>();
   ColLaunch L;
-  L.grid = dim3((unsigned)((a.W + C - 1) / C), (unsigned)n_outer, (unsigned)std::max(1, a.nb));
+  const unsigned ncb = (unsigned)((a.W + C - 1) / C);
+  L.grid = a.outer_x ? dim3((unsigned)n_outer, ncb, (unsigned)std::max(1, a.nb))
+                     : dim3(ncb, (unsigned)n_outer, (unsigned)std::max(1, a.nb));
   L.stream = e->stream;
   L.in = in;
   L.out = out;
@@ -430,6 +433,9 @@ bool launch_col(clat_engine* e, const PassCfg& pc, const P2Args& a, int n_outer,
   c.mask_os = a.mask_os;
   c.in_bs = a.in_bs;
   c.out_bs = a.out_bs;
+  c.in_cbs = a.in_cbs ? a.in_cbs : C;
+  c.out_cbs = a.out_cbs ? a.out_cbs : C;
+  c.outer_x = a.outer_x;
   L.T2L = sizeof(TC) == 8 ? (const void*)e->dtw[axis] : (const void*)e->dtw32[axis];
   L.G = gc_of<TC>(e);
   constexpr int combo = combo_of<TI, TC, TO>();
@@ -453,7 +459,8 @@ void launch2(clat_engine* e, const PassCfg& pc, P2Args a, int n_outer, int axis,
   if constexpr (!LINE) {
     if (launch_col<TI, TC, TO, MODE>(e, pc, a, n_outer, axis, in, out, kind, bytes)) return;
   }
-  if (a.rowmask || a.nb > 1) throw host::Error(CLAT_EDEVICE, "row mask or batch on the general pass kernel");
+  if (a.rowmask || a.nb > 1 || a.in_cbs || a.out_cbs || a.outer_x)
+    throw host::Error(CLAT_EDEVICE, "row mask, batch or tiled plane on the general pass kernel");
   a.R1 = pc.R1;
   a.R2 = pc.R2;
   a.R1p = pc.R1p;
@@ -628,12 +635,26 @@ void prepare_fast(clat_engine* e, const double2* sites_in, double2* sites_out) {
 
   if (e->shared()) {
     const long long W = 2LL * Xi;
-    // B holds the y-forward output as [fy][z][fx]: the z pass reads rows W apart
+    // B holds the y-forward output as [fy][z][fx] (the z pass reads rows W
+    // apart) or, with e->btile, as [fy][fx / C][z][fx % C]: element (fy, z)
+    // of column strip j at fy Z W + j Z C + z C
+    const bool bt = e->btile;
+    const long long Cb = bt ? e->gc_c : 0;
+    if (bt && (col_c<TC>() != Cb || col_c<TCZ>() != Cb || col_c<TCY>() != Cb))
+      throw host::Error(CLAT_EDEVICE, "tiled B with column passes of different widths");
+    const long long bz = bt ? Cb : W;       // B: z stride
+    const long long bcs = bt ? Zi * Cb : 0; // B: column-strip stride (0: C)
+    // with strips the y inverse runs z fastest, so the blocks in flight read
+    // whole strips (y inverse 2.16 ms per step against 2.25 column-block
+    // fastest; the y forward, which writes them, measured the other way round:
+    // 2.64 against 2.48)
+    const int yo = bt ? 1 : 0;
     xfwd(2, 0, 0, 0, e->A);
     {
       P2Args a = base_args(e);
       a.L = Yi; a.nsub = 2; a.W = (int)W;
-      a.in_os = (long long)Yi * W; a.in_is = W; a.out_os = W; a.out_is = (long long)Zi * W;
+      a.in_os = (long long)Yi * W; a.in_is = W; a.out_os = bz; a.out_is = (long long)Zi * W;
+      a.out_cbs = bcs;
       if (lk) { a.rowmask = e->d_linemask; a.mask_os = e->cyf.R1; batch(a, (long long)e->a_set, (long long)e->b_set); }
       launch2<TA, TC, TB, PM_FWD, false>(e, e->cyf, a, Zi, 1, e->A, e->B, K_YFWD,
                                          (sa * 2 * live + sb * 4) * N * nb);
@@ -641,7 +662,8 @@ void prepare_fast(clat_engine* e, const double2* sites_in, double2* sites_out) {
     {
       P2Args a = base_args(e);
       a.L = Zi; a.nsub = 2; a.W = (int)W;
-      a.in_os = a.out_os = (long long)Zi * W; a.in_is = a.out_is = W;
+      a.in_os = a.out_os = (long long)Zi * W; a.in_is = a.out_is = bz;
+      a.in_cbs = a.out_cbs = bcs;
       a.gxm = 1; a.gym = 1;
       if (lk) batch(a, (long long)e->b_set, (long long)e->b_set);
       launch2<TB, TCZ, TB, PM_CONV, false>(e, e->czc, a, 2 * Yi, 2, e->B, e->B, K_ZCONV,
@@ -650,7 +672,9 @@ void prepare_fast(clat_engine* e, const double2* sites_in, double2* sites_out) {
     {
       P2Args a = base_args(e);
       a.L = Yi; a.nsub = 2; a.W = (int)W;
-      a.in_os = W; a.in_is = (long long)Zi * W; a.out_os = (long long)Yi * W; a.out_is = W;
+      a.in_os = bz; a.in_is = (long long)Zi * W; a.out_os = (long long)Yi * W; a.out_is = W;
+      a.in_cbs = bcs;
+      a.outer_x = yo;
       if (lk) { a.rowmask = e->d_linemask; a.mask_os = e->cyi.R1; batch(a, (long long)e->b_set, (long long)e->d_set); }
       launch2<TB, TCY, TD, PM_INV, false>(e, e->cyi, a, Zi, 1, e->B, e->D, K_YINV,
                                           (sb * 4 + sd * 2 * live) * N * nb);
@@ -1251,6 +1275,17 @@ int clat_create(int X, int Y, int Z, int precision, int schedule, clat_engine** 
       }
       if (tcz == flt) e->Gc32 = (float*)e->dalloc(e->gc_elems * sizeof(float));
       else e->Gc64 = (double*)e->dalloc(e->gc_elems * sizeof(double));
+      // B as strips of C columns, each Z rows deep ([fy][strip][z][C]): the conv
+      // pass then streams one contiguous 16 KB tile per block instead of
+      // 128-byte row pieces 2X apart (a copy with the two access maps: 5.96 vs
+      // 4.98 TB/s); the y passes keep their access pattern.  Needs all three
+      // passes on column kernels with one column count C.
+      const long long W = 2LL * e->Xi;
+      e->btile = e->shared() && e->conv_tiled && tc == tcz && tcy == tcz && W % e->gc_c == 0 &&
+                 col_pair_compiled(e->cyf.R1, e->cyf.R2) && col_pair_compiled(e->cyi.R1, e->cyi.R2) &&
+                 col_fits(e->Yi, 2, W, e->s_a, (long long)e->Zi * W, e->s_b) &&
+                 col_fits(e->Yi, 2, (long long)e->Zi * W, e->s_b, W, e->s_d);
+      if (const char* g = std::getenv("CLAT_BTILE")) e->btile = e->btile && std::atoi(g) != 0;
     }
     const size_t N = e->N();
     e->f_bytes = N * e->s_in;

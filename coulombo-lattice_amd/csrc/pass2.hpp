@@ -47,6 +47,9 @@ struct P2Args {
   // batched prepares (column kernel only): count and element strides
   int nb;
   long long in_bs, out_bs;
+  // column kernels only: element stride between blocks of C columns (0: C, row-major plane)
+  long long in_cbs, out_cbs;
+  int outer_x;  // column kernels only: outer index on blockIdx.x
 };
 
 // register DFT of size R: compile-time when RT > 0, else dispatched on R

@@ -71,11 +71,13 @@ def test_prepare_matches_oracle(prec, sched, dims):
 
 @pytest.mark.parametrize("prec,sched", COMBOS, ids=IDS)
 @pytest.mark.parametrize("dims", [(3, 32, 64), (2, 45, 96), (3, 120, 160), (2, 192, 256), (4, 256, 128),
-                                  (2, 400, 8), (17, 64, 45)])
+                                  (2, 400, 8), (17, 64, 45), (8, 45, 96), (24, 120, 160), (16, 32, 400)])
 def test_column_kernels_match_oracle(prec, sched, dims):
     """y and z lengths on every compile-time radix pair of the column passes
     (32, 45, 64, 96, 120, 128, 160, 192, 256, 400), with fewer columns than
-    one block holds (X = 2, 3) and a ragged last block (X = 17)."""
+    one block holds (X = 2, 3), a ragged last block (X = 17), and 2X a
+    multiple of the column count (X = 8, 16, 24: the y-forward output is then
+    stored in column strips, which the z pass reads as contiguous tiles)."""
     X, Y, Z = dims
     print("case column", dims, prec, sched, flush=True)
     rng = np.random.default_rng(X * 1000003 + Y * 1009 + Z)
@@ -219,6 +221,12 @@ def test_batched_prepares(prec, monkeypatch):
     monkeypatch.setenv("CLAT_BATCH", "1")
     one = run()
     assert np.array_equal(got, one)
+    # the y-forward output in column strips (the default on this grid) or in
+    # plain rows: same arithmetic, other addresses
+    monkeypatch.setenv("CLAT_BTILE", "0")
+    rows = run()
+    assert np.array_equal(got, rows)
+    monkeypatch.delenv("CLAT_BTILE")
     # the five diagonal densities go two to a prepare (real and imaginary
     # part); one prepare each gives the same values to rounding
     monkeypatch.setenv("CLAT_PACK", "0")
