@@ -231,6 +231,7 @@ struct clat_engine {
   bool colk = true;             // compile-time column kernels for the y and z passes
   bool conv_tiled = false;      // conv pass on the column kernel (tiled kernel table)
   bool btile = false;           // B in strips of C columns x Z rows (shared schedule)
+  bool atile = false;           // A and D in strips of C columns x Y rows (atom path, with btile)
   int gc_c = 16, gc_ncb = 1, gc_npar = 1, gc_w = 0;  // kernel-table tiling
   double* G64 = nullptr;  // Ghat octant (Z+1)(Y+1)(X+1), FP64
   float* G32 = nullptr;   // same, FP32 (generic complex64 kernels)
@@ -517,7 +518,19 @@ void launch_line(clat_engine* e, const void* in, void* out, long long bs, int ki
     a.site_in2[b] = MODE == PM_FWD && b < e->cur_nb ? e->b_in2[b] : nullptr;
     a.site_out2[b] = MODE == PM_INV && b < e->cur_nb ? e->b_out2[b] : nullptr;
   }
-  a.in_ls = a.out_ls = 2LL * e->Xi;
+  if ((size_t)e->a_set > (size_t)INT32_MAX) throw host::Error(CLAT_EUNSUP, "grid too large for the line kernels");
+  const int W = 2 * e->Xi;
+  a.Yl = e->Yi;
+  a.lz = e->Yi * W;
+  if (e->atile) {
+    a.ly = e->gc_c;
+    a.lgsw = log2i(e->gc_c);
+    a.strip = e->Yi * e->gc_c;
+  } else {
+    a.ly = W;
+    a.lgsw = 0;
+    a.strip = 0;
+  }
   a.in_bs = a.out_bs = bs;
   L.T2L = sizeof(TC) == 8 ? (const void*)e->dtw[0] : (const void*)e->dtw32[0];
   constexpr int combo = combo_of<TI, TC, TO>();
@@ -644,6 +657,8 @@ void prepare_fast(clat_engine* e, const double2* sites_in, double2* sites_out) {
       throw host::Error(CLAT_EDEVICE, "tiled B with column passes of different widths");
     const long long bz = bt ? Cb : W;       // B: z stride
     const long long bcs = bt ? Zi * Cb : 0; // B: column-strip stride (0: C)
+    // A and D in strips ([z][fx / C][y][fx % C]) on the line-kernel path
+    const bool at = bt && lk && e->atile;
     // with strips the y inverse runs z fastest, so the blocks in flight read
     // whole strips (y inverse 2.16 ms per step against 2.25 column-block
     // fastest; the y forward, which writes them, measured the other way round:
@@ -655,6 +670,7 @@ void prepare_fast(clat_engine* e, const double2* sites_in, double2* sites_out) {
       a.L = Yi; a.nsub = 2; a.W = (int)W;
       a.in_os = (long long)Yi * W; a.in_is = W; a.out_os = bz; a.out_is = (long long)Zi * W;
       a.out_cbs = bcs;
+      if (at) { a.in_is = Cb; a.in_cbs = (long long)Yi * Cb; }
       if (lk) { a.rowmask = e->d_linemask; a.mask_os = e->cyf.R1; batch(a, (long long)e->a_set, (long long)e->b_set); }
       launch2<TA, TC, TB, PM_FWD, false>(e, e->cyf, a, Zi, 1, e->A, e->B, K_YFWD,
                                          (sa * 2 * live + sb * 4) * N * nb);
@@ -674,6 +690,7 @@ void prepare_fast(clat_engine* e, const double2* sites_in, double2* sites_out) {
       a.L = Yi; a.nsub = 2; a.W = (int)W;
       a.in_os = bz; a.in_is = (long long)Zi * W; a.out_os = (long long)Yi * W; a.out_is = W;
       a.in_cbs = bcs;
+      if (at) { a.out_is = Cb; a.out_cbs = (long long)Yi * Cb; }
       a.outer_x = yo;
       if (lk) { a.rowmask = e->d_linemask; a.mask_os = e->cyi.R1; batch(a, (long long)e->b_set, (long long)e->d_set); }
       launch2<TB, TCY, TD, PM_INV, false>(e, e->cyi, a, Zi, 1, e->B, e->D, K_YINV,
@@ -1286,6 +1303,14 @@ int clat_create(int X, int Y, int Z, int precision, int schedule, clat_engine** 
                  col_fits(e->Yi, 2, W, e->s_a, (long long)e->Zi * W, e->s_b) &&
                  col_fits(e->Yi, 2, (long long)e->Zi * W, e->s_b, W, e->s_d);
       if (const char* g = std::getenv("CLAT_BTILE")) e->btile = e->btile && std::atoi(g) != 0;
+      // A and D likewise ([z][fx / C][y][fx % C]) where the x passes run on the
+      // line kernels: the y forward reads and the y inverse writes contiguous
+      // strips, the line kernels write and read 128-byte pieces of them (a
+      // copy with the y-forward access map: 4.66 against 3.57 TB/s; with the
+      // x-forward write map: 6.36 against 4.56)
+      e->atile = e->btile && e->xline && col_fits(e->Yi, 2, e->gc_c, e->s_a, (long long)e->Zi * W, e->s_b) &&
+                 col_fits(e->Yi, 2, (long long)e->Zi * W, e->s_b, e->gc_c, e->s_d);
+      if (const char* g = std::getenv("CLAT_ATILE")) e->atile = e->atile && std::atoi(g) != 0;
     }
     const size_t N = e->N();
     e->f_bytes = N * e->s_in;

@@ -32,9 +32,36 @@ struct LineArgs {
   // splits back into site_out[b] = (Re, 0) and site_out2[b] = (Im, 0)
   const double2* site_in2[LINE_NB];
   double2* site_out2[LINE_NB];
-  long long in_ls, out_ls;          // element stride between lines of the dense side
+  // dense side (A forward, D inverse; line id = z Yl + y, 2X frequencies f per
+  // line): element z lz + y ly + f in plain rows (lgsw = 0), or in strips of
+  // sw = 2^lgsw frequencies, z lz + y ly + (f / sw) strip + f % sw
+  // (offsets within one prepare's dense side stay below 2^31)
+  int Yl, lgsw;
+  int lz, ly, strip;
   long long in_bs, out_bs;          // element stride between batched prepares on the dense side
 };
+
+// element offset of (line, first frequency) and of frequency f within the line
+__device__ __forceinline__ int line_base(const LineArgs& a, int line) {
+  const int z = line / a.Yl;
+  return z * a.lz + (line - z * a.Yl) * a.ly;
+}
+__device__ __forceinline__ int freq_off(const LineArgs& a, int f) {
+  return a.lgsw ? (f >> a.lgsw) * a.strip + (f & ((1 << a.lgsw) - 1)) : f;
+}
+// offsets of frequencies f0 + k1 P, k1 < R1: linear in k1 when P is a multiple
+// of the strip width (or in plain rows), else one split per frequency
+template <int R1, int P>
+__device__ __forceinline__ void freq_offs(const LineArgs& a, int f0, int* off) {
+  if (a.lgsw == 0 || (P & ((1 << a.lgsw) - 1)) == 0) {
+    const int base = freq_off(a, f0), step = a.lgsw ? (P >> a.lgsw) * a.strip : P;
+#pragma unroll
+    for (int k1 = 0; k1 < R1; ++k1) off[k1] = base + k1 * step;
+  } else {
+#pragma unroll
+    for (int k1 = 0; k1 < R1; ++k1) off[k1] = freq_off(a, f0 + k1 * P);
+  }
+}
 
 template <int C, int R1, int R2, int NSUB>
 constexpr int line_threads() {
@@ -123,9 +150,11 @@ __global__ __launch_bounds__((line_threads<C, R1, R2, NSUB>())) void k_linepass(
 #pragma unroll
       for (int m = 0; m < R1; ++m) w[m] = lds[xi(c2, s2, k2, m)];
       rdft<R1, false, TC>(w);
-      C2<TO>* o = out + (size_t)blockIdx.y * a.out_bs + (size_t)a.lines[b0 + c2] * a.out_ls + k2 * NSUB + s2;
+      C2<TO>* o = out + (size_t)blockIdx.y * a.out_bs + line_base(a, a.lines[b0 + c2]);
+      int off[R1];
+      freq_offs<R1, R2 * NSUB>(a, k2 * NSUB + s2, off);
 #pragma unroll
-      for (int k1 = 0; k1 < R1; ++k1) o[k1 * R2 * NSUB] = cvt<TC, TO>(w[k1]);
+      for (int k1 = 0; k1 < R1; ++k1) o[off[k1]] = cvt<TC, TO>(w[k1]);
     }
   } else {
     // ---------------------------------------------------------- P2 <- D
@@ -133,9 +162,11 @@ __global__ __launch_bounds__((line_threads<C, R1, R2, NSUB>())) void k_linepass(
     // costs more than their loads)
     if (ok2) {
       CT w[R1];
-      const C2<TI>* ip = in + (size_t)blockIdx.y * a.in_bs + (size_t)a.lines[b0 + c2] * a.in_ls + k2 * NSUB + s2;
+      const C2<TI>* ip = in + (size_t)blockIdx.y * a.in_bs + line_base(a, a.lines[b0 + c2]);
+      int off[R1];
+      freq_offs<R1, R2 * NSUB>(a, k2 * NSUB + s2, off);
 #pragma unroll
-      for (int k1 = 0; k1 < R1; ++k1) w[k1] = cvt<TI, TC>(ldg_stream(ip + k1 * R2 * NSUB));
+      for (int k1 = 0; k1 < R1; ++k1) w[k1] = cvt<TI, TC>(ldg_stream(ip + off[k1]));
       rdft<R1, true, TC>(w);
       const uint32_t tstep = (uint32_t)(2 * k2 + s2) * SC;
 #pragma unroll

@@ -221,12 +221,16 @@ def test_batched_prepares(prec, monkeypatch):
     monkeypatch.setenv("CLAT_BATCH", "1")
     one = run()
     assert np.array_equal(got, one)
-    # the y-forward output in column strips (the default on this grid) or in
-    # plain rows: same arithmetic, other addresses
+    # the x-forward, y-forward and y-inverse outputs in column strips (the
+    # default on this grid) or in plain rows: same arithmetic, other addresses
+    monkeypatch.setenv("CLAT_ATILE", "0")
+    rows_ad = run()
+    assert np.array_equal(got, rows_ad)
     monkeypatch.setenv("CLAT_BTILE", "0")
     rows = run()
     assert np.array_equal(got, rows)
     monkeypatch.delenv("CLAT_BTILE")
+    monkeypatch.delenv("CLAT_ATILE")
     # the five diagonal densities go two to a prepare (real and imaginary
     # part); one prepare each gives the same values to rounding
     monkeypatch.setenv("CLAT_PACK", "0")
