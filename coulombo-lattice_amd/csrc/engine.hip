@@ -661,8 +661,8 @@ void prepare_fast(clat_engine* e, const double2* sites_in, double2* sites_out) {
     const bool at = bt && lk && e->atile;
     // with strips the y inverse runs z fastest, so the blocks in flight read
     // whole strips (y inverse 2.16 ms per step against 2.25 column-block
-    // fastest; the y forward, which writes them, measured the other way round:
-    // 2.64 against 2.48)
+    // fastest; with A in plain rows the y forward measured the other way
+    // round, 2.64 against 2.48)
     const int yo = bt ? 1 : 0;
     xfwd(2, 0, 0, 0, e->A);
     {
@@ -670,7 +670,9 @@ void prepare_fast(clat_engine* e, const double2* sites_in, double2* sites_out) {
       a.L = Yi; a.nsub = 2; a.W = (int)W;
       a.in_os = (long long)Yi * W; a.in_is = W; a.out_os = bz; a.out_is = (long long)Zi * W;
       a.out_cbs = bcs;
-      if (at) { a.in_is = Cb; a.in_cbs = (long long)Yi * Cb; }
+      // with A in strips too, the y forward also walks z fastest (1.98 ms per
+      // step against 2.12; the z conv after it 3.81 against 3.90)
+      if (at) { a.in_is = Cb; a.in_cbs = (long long)Yi * Cb; a.outer_x = 1; }
       if (lk) { a.rowmask = e->d_linemask; a.mask_os = e->cyf.R1; batch(a, (long long)e->a_set, (long long)e->b_set); }
       launch2<TA, TC, TB, PM_FWD, false>(e, e->cyf, a, Zi, 1, e->A, e->B, K_YFWD,
                                          (sa * 2 * live + sb * 4) * N * nb);
