@@ -1042,10 +1042,14 @@ void build_products(clat_engine* e) {
   }
   e->rho_states = e->n_states;
   if (e->n_sites == 0) return;
-  // S sites per block (power of two, <= 64): all states of S + 1 sites in about 44 KB of LDS
+  // S sites per block (power of two, <= 64): all states of S + 1 sites in at
+  // most 22.5 KB of LDS, so that six blocks share a CU (8 states: 0.59 ms per
+  // step against 0.72 with 44 KB tiles), but at least two sites, or too few
+  // threads load (32 states: 2.96 ms against 4.10 with one site)
   const size_t per_site = (size_t)e->n_states * e->n_orb * sizeof(double2);
   int S = 64;
-  while (S > 1 && (size_t)(S + 1) * per_site > 45056) S /= 2;
+  while (S > 1 && (size_t)(S + 1) * per_site > 23040) S /= 2;
+  if (S == 1 && 3 * per_site <= 45056) S = 2;
   e->kbegin();
   if ((size_t)(S + 1) * per_site <= 65536) {
     const unsigned nb = (unsigned)((e->n_sites + S - 1) / S);
