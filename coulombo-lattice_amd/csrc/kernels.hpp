@@ -558,7 +558,7 @@ __global__ void k_products_direct(const double2* __restrict__ coeffs, int64_t n_
 // Stage 1: a 48x48 (slot, product) tile per block, 3x3 pairs per thread, over
 // one chunk of sites; partial sums to part[chunk][l][r][4].
 // ---------------------------------------------------------------------------
-constexpr int OV_S = 32;   // sites staged per step
+constexpr int OV_S = 16;   // sites staged per step
 
 // TX x TX threads, 3 x 3 (left, right) pairs each: a tile edge of 3 TX.
 template <int TX>
@@ -608,7 +608,7 @@ __global__ __launch_bounds__(TX * TX) void k_overlap_partial(const double2* __re
     }
     __syncthreads();
     if (sb + OV_S < s1) fetch(sb + OV_S);
-#pragma unroll 4
+#pragma unroll 2
     for (int s = 0; s < OV_S; ++s) {
       double2 v[3], r[3];
 #pragma unroll
