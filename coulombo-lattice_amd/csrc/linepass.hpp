@@ -169,10 +169,25 @@ __global__ __launch_bounds__((line_threads<C, R1, R2, NSUB>())) void k_linepass(
       for (int k1 = 0; k1 < R1; ++k1) w[k1] = cvt<TI, TC>(ldg_stream(ip + off[k1]));
       rdft<R1, true, TC>(w);
       const uint32_t tstep = (uint32_t)(2 * k2 + s2) * SC;
+      // twiddles W_2L^{m q}, q = 2 k2 + s2.  FP64: the powers of two m come
+      // from the table and every other m is the product of its highest power
+      // of two and the rest (at most three products deep), so a thread loads
+      // 4 sixteen-byte twiddles instead of R1 - 1, which through the L1 would
+      // outweigh the pass's own data
+      CT tw[R1];
+#pragma unroll
+      for (int m = 1; m < R1; ++m) {
+        if (sizeof(TC) == 4 || (m & (m - 1)) == 0) {
+          tw[m] = ldg_off(T2L, (uint32_t)m * tstep);
+        } else {
+          const int hi = 1 << (31 - __builtin_clz((unsigned)m));
+          tw[m] = cmul<TC>(tw[hi], tw[m - hi]);
+        }
+      }
 #pragma unroll
       for (int m = 0; m < R1; ++m) {
         CT t = w[m];
-        if (m > 0) t = cmulc<TC>(t, ldg_off(T2L, (uint32_t)m * tstep));
+        if (m > 0) t = cmulc<TC>(t, tw[m]);
         lds[xi(c2, s2, k2, m)] = t;
       }
     }
