@@ -191,12 +191,16 @@ __global__ __launch_bounds__((col_threads<C, R1>())) void k_colpass(const C2<TI>
     for (int n2 = 0; n2 < R2; ++n2) acc[n2] = zero;
   }
 
-  // conv pass: the block's kernel-table tile, C values per rz row
+  // conv pass: the block's kernel-table tile, C values per rz row.  The tile
+  // pointer stays block-uniform and the column goes into the 32-bit offset,
+  // so the loads take a scalar base (no per-thread 64-bit address arithmetic)
   const TC* Gb = nullptr;
+  uint32_t gcol = 0;
   if constexpr (MODE == PM_CONV) {
     const int fy = o * a.gym + a.gya;
     const int ry = min(fy, a.Y2 - fy);
-    Gb = G + ((size_t)(ry * a.npar + a.par) * a.ncb + cbk) * a.GZ1 * C + c;
+    Gb = G + ((size_t)(ry * a.npar + a.par) * a.ncb + cbk) * a.GZ1 * C;
+    gcol = (uint32_t)c * (uint32_t)sizeof(TC);
   }
 
 #pragma unroll
@@ -255,7 +259,7 @@ __global__ __launch_bounds__((col_threads<C, R1>())) void k_colpass(const C2<TI>
               const int fz = 2 * R2 * k1 + q;
               off = (uint32_t)min(fz, 2 * L - fz) * RB;
             }
-            v[k1] = cscale<TC>(v[k1], ldg_off(Gb, off));
+            v[k1] = cscale<TC>(v[k1], ldg_off(Gb, off + gcol));
           }
         }
         rdft<R1, true, TC>(v);  // v[n1] = sum_k1 X W_R1^{-n1 k1}
