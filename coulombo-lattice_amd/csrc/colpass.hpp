@@ -69,9 +69,53 @@ __device__ __forceinline__ T ldg_stream(const T* p) {
   }
   return r;
 }
+// Column-pass memory access from a block-uniform base: a buffer resource in
+// scalar registers with the byte offset as the lane offset, so the lanes need
+// no 64-bit address arithmetic (a nontemporal global load takes a full
+// per-lane address).  The resource covers COL_RANGE bytes; a lane that must
+// not touch memory (a row of an empty x-line, a column past the plane) passes
+// COL_OOB, which is out of range: the load returns zero and the store is
+// dropped by the hardware, with no branch around the instruction.  Offsets of
+// real elements stay below COL_RANGE (the host checks rows x stride + 4 KiB
+// < 2^32).
+constexpr uint32_t COL_RANGE = 0xFFFFFF00u;
+constexpr uint32_t COL_OOB = 0xFFFFFFF0u;
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t col_rsrc(const void* base) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0, (int)COL_RANGE, 0x00020000);
+}
+// streaming load (nontemporal cache policy)
 template <typename T>
 __device__ __forceinline__ T ldg_stream_off(const T* base, uint32_t byte_off) {
-  return ldg_stream(reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + byte_off));
+  static_assert(sizeof(T) == 8 || sizeof(T) == 16, "complex64 or complex128 element");
+  const __amdgpu_buffer_rsrc_t rs = col_rsrc(base);
+  constexpr int NT_POLICY = 2;  // nt
+  T r;
+  if constexpr (sizeof(T) == 8) {
+    typedef int i2 __attribute__((ext_vector_type(2)));
+    const i2 u = __builtin_amdgcn_raw_buffer_load_b64(rs, (int)byte_off, 0, NT_POLICY);
+    __builtin_memcpy(&r, &u, 8);
+  } else {
+    typedef int i4 __attribute__((ext_vector_type(4)));
+    const i4 u = __builtin_amdgcn_raw_buffer_load_b128(rs, (int)byte_off, 0, NT_POLICY);
+    __builtin_memcpy(&r, &u, 16);
+  }
+  return r;
+}
+template <typename T>
+__device__ __forceinline__ void stg_buf(T* base, uint32_t byte_off, T v) {
+  static_assert(sizeof(T) == 8 || sizeof(T) == 16, "complex64 or complex128 element");
+  const __amdgpu_buffer_rsrc_t rs = col_rsrc(base);
+  if constexpr (sizeof(T) == 8) {
+    typedef int i2 __attribute__((ext_vector_type(2)));
+    i2 u;
+    __builtin_memcpy(&u, &v, 8);
+    __builtin_amdgcn_raw_buffer_store_b64(u, rs, (int)byte_off, 0, 0);
+  } else {
+    typedef int i4 __attribute__((ext_vector_type(4)));
+    i4 u;
+    __builtin_memcpy(&u, &v, 16);
+    __builtin_amdgcn_raw_buffer_store_b128(u, rs, (int)byte_off, 0, 0);
+  }
 }
 
 // v * W_R^{e} (forward sign) or v * W_R^{-e} (CONJ), W_R = exp(-2 pi i / R), e compile-time
@@ -160,20 +204,24 @@ __global__ __launch_bounds__((col_threads<C, R1>())) void k_colpass(const C2<TI>
     const uint32_t live = (MODE == PM_FWD && a.rowmask && ok) ? a.rowmask[o * a.mask_os + r] : ~0u;
 #pragma unroll
     for (int n2 = 0; n2 < R2; ++n2) {
-      const uint32_t off = rbase + (uint32_t)n2 * a.in_is_hi;
-      x[n2] = (ok && ((live >> n2) & 1u)) ? cvt<TI, TC>(ldg_stream_off(bin, off)) : zero;
+      const uint32_t off = (ok && ((live >> n2) & 1u)) ? rbase + (uint32_t)n2 * a.in_is_hi : COL_OOB;
+      x[n2] = cvt<TI, TC>(ldg_stream_off(bin, off));
     }
   }
   // INV: thread (c, k2 = r) loads X_s[R2 k1 + k2] of every sub-line up front
   CT xi[MODE == PM_INV ? NSUB * R1 : 1];
   if constexpr (MODE == PM_INV) {
     const bool ok = okc && r < R2;
+    // row (R2 k1 + r) NSUB + s: one per-thread base and block-uniform steps
+    // (a multiply per row would cost a 64-bit integer multiply-add each)
+    const uint32_t rb = (uint32_t)(r * NSUB) * a.in_is_lo + cin;
+    const uint32_t k1s = (uint32_t)(R2 * NSUB) * a.in_is_lo;
 #pragma unroll
     for (int s = 0; s < NSUB; ++s)
 #pragma unroll
       for (int k1 = 0; k1 < R1; ++k1) {
-        const uint32_t off = (uint32_t)((R2 * k1 + r) * NSUB + s) * a.in_is_lo + cin;
-        xi[s * R1 + k1] = ok ? cvt<TI, TC>(ldg_stream_off(bin, off)) : zero;
+        const uint32_t off = ok ? rb + (uint32_t)k1 * k1s + (uint32_t)s * a.in_is_lo : COL_OOB;
+        xi[s * R1 + k1] = cvt<TI, TC>(ldg_stream_off(bin, off));
       }
   }
 
@@ -237,11 +285,10 @@ __global__ __launch_bounds__((col_threads<C, R1>())) void k_colpass(const C2<TI>
       }
       if constexpr (MODE == PM_FWD) {
         if (okc) {
+          const uint32_t rb = (uint32_t)(r * NSUB + s) * a.out_is_lo + cout;
+          const uint32_t k1s = (uint32_t)(R2 * NSUB) * a.out_is_lo;
 #pragma unroll
-          for (int k1 = 0; k1 < R1; ++k1) {
-            const uint32_t off = (uint32_t)((R2 * k1 + r) * NSUB + s) * a.out_is_lo + cout;
-            stg_off(bout, off, cvt<TC, TO>(v[k1]));
-          }
+          for (int k1 = 0; k1 < R1; ++k1) stg_off(bout, rb + (uint32_t)k1 * k1s, cvt<TC, TO>(v[k1]));
         }
       } else {
         if constexpr (MODE == PM_CONV) {
@@ -290,14 +337,13 @@ __global__ __launch_bounds__((col_threads<C, R1>())) void k_colpass(const C2<TI>
 
   // ------------------------------------------------------------------ store
   if constexpr (MODE != PM_FWD) {
-    if (okc && r < R1) {
-      const uint32_t rbase = (uint32_t)r * a.out_is_lo + cout;
-      const uint32_t live = (MODE == PM_INV && a.rowmask) ? a.rowmask[o * a.mask_os + r] : ~0u;
+    const bool ok = okc && r < R1;
+    const uint32_t rbase = (uint32_t)r * a.out_is_lo + cout;
+    const uint32_t live = (MODE == PM_INV && a.rowmask && ok) ? a.rowmask[o * a.mask_os + r] : ~0u;
 #pragma unroll
-      for (int n2 = 0; n2 < R2; ++n2) {
-        const uint32_t off = rbase + (uint32_t)n2 * a.out_is_hi;
-        if ((live >> n2) & 1u) stg_off(bout, off, cvt<TC, TO>(acc[n2]));
-      }
+    for (int n2 = 0; n2 < R2; ++n2) {
+      const uint32_t off = (ok && ((live >> n2) & 1u)) ? rbase + (uint32_t)n2 * a.out_is_hi : COL_OOB;
+      stg_buf(bout, off, cvt<TC, TO>(acc[n2]));
     }
   }
 }

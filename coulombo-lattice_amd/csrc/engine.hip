@@ -390,10 +390,11 @@ bool col_pair_compiled(int R1, int R2) {
   }
 }
 
-// row offsets of a column pass stay below 2^32 bytes
+// row offsets of a column pass stay below the 2^32 - 256 bytes its buffer
+// resource covers (4 KiB of margin for the column offset)
 bool col_fits(int L, int nsub, long long in_is, size_t si, long long out_is, size_t so) {
-  const double lim = 4294967295.0, rows = (double)L * nsub;
-  return rows * (double)in_is * (double)si + 16.0 * 16 < lim && rows * (double)out_is * (double)so + 16.0 * 16 < lim;
+  const double lim = 4294967296.0 - 4096.0, rows = (double)L * nsub;
+  return rows * (double)in_is * (double)si < lim && rows * (double)out_is * (double)so < lim;
 }
 
 // Column pass on the compile-time column kernel; false when the radix pair
