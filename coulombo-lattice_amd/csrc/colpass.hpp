@@ -78,6 +78,11 @@ __device__ __forceinline__ T ldg_stream(const T* p) {
 // dropped by the hardware, with no branch around the instruction.  Offsets of
 // real elements stay below COL_RANGE (the host checks rows x stride + 4 KiB
 // < 2^32).
+// a block-uniform product (row index x row stride) kept in a scalar register:
+// the compiler would otherwise fold it with the lane offset into a 64-bit
+// integer multiply-add per row
+__device__ __forceinline__ uint32_t suni(uint32_t v) { return (uint32_t)__builtin_amdgcn_readfirstlane((int)v); }
+
 constexpr uint32_t COL_RANGE = 0xFFFFFF00u;
 constexpr uint32_t COL_OOB = 0xFFFFFFF0u;
 __device__ __forceinline__ __amdgpu_buffer_rsrc_t col_rsrc(const void* base) {
@@ -204,7 +209,7 @@ __global__ __launch_bounds__((col_threads<C, R1>())) void k_colpass(const C2<TI>
     const uint32_t live = (MODE == PM_FWD && a.rowmask && ok) ? a.rowmask[o * a.mask_os + r] : ~0u;
 #pragma unroll
     for (int n2 = 0; n2 < R2; ++n2) {
-      const uint32_t off = (ok && ((live >> n2) & 1u)) ? rbase + (uint32_t)n2 * a.in_is_hi : COL_OOB;
+      const uint32_t off = (ok && ((live >> n2) & 1u)) ? rbase + suni((uint32_t)n2 * a.in_is_hi) : COL_OOB;
       x[n2] = cvt<TI, TC>(ldg_stream_off(bin, off));
     }
   }
@@ -220,7 +225,7 @@ __global__ __launch_bounds__((col_threads<C, R1>())) void k_colpass(const C2<TI>
     for (int s = 0; s < NSUB; ++s)
 #pragma unroll
       for (int k1 = 0; k1 < R1; ++k1) {
-        const uint32_t off = ok ? rb + (uint32_t)k1 * k1s + (uint32_t)s * a.in_is_lo : COL_OOB;
+        const uint32_t off = ok ? rb + suni((uint32_t)k1 * k1s + (uint32_t)s * a.in_is_lo) : COL_OOB;
         xi[s * R1 + k1] = cvt<TI, TC>(ldg_stream_off(bin, off));
       }
   }
@@ -288,7 +293,9 @@ __global__ __launch_bounds__((col_threads<C, R1>())) void k_colpass(const C2<TI>
           const uint32_t rb = (uint32_t)(r * NSUB + s) * a.out_is_lo + cout;
           const uint32_t k1s = (uint32_t)(R2 * NSUB) * a.out_is_lo;
 #pragma unroll
-          for (int k1 = 0; k1 < R1; ++k1) stg_off(bout, rb + (uint32_t)k1 * k1s, cvt<TC, TO>(v[k1]));
+          for (int k1 = 0; k1 < R1; ++k1) {
+            stg_off(bout, rb + suni((uint32_t)k1 * k1s), cvt<TC, TO>(v[k1]));
+          }
         }
       } else {
         if constexpr (MODE == PM_CONV) {
@@ -342,7 +349,7 @@ __global__ __launch_bounds__((col_threads<C, R1>())) void k_colpass(const C2<TI>
     const uint32_t live = (MODE == PM_INV && a.rowmask && ok) ? a.rowmask[o * a.mask_os + r] : ~0u;
 #pragma unroll
     for (int n2 = 0; n2 < R2; ++n2) {
-      const uint32_t off = (ok && ((live >> n2) & 1u)) ? rbase + (uint32_t)n2 * a.out_is_hi : COL_OOB;
+      const uint32_t off = (ok && ((live >> n2) & 1u)) ? rbase + suni((uint32_t)n2 * a.out_is_hi) : COL_OOB;
       stg_buf(bout, off, cvt<TC, TO>(acc[n2]));
     }
   }
