@@ -43,6 +43,9 @@ struct ColArgs2 {
   // index, 1 = the reverse (outer index fastest, so the blocks in flight share
   // column blocks)
   int outer_x;
+  // > 0: the batched prepares are folded into blockIdx.x = prepare * nxb +
+  // column block (nxb column blocks), instead of blockIdx.z
+  int nxb;
 };
 
 template <typename T>
@@ -172,7 +175,13 @@ __global__ __launch_bounds__((col_threads<C, R1>())) void k_colpass(const C2<TI>
   const int tid = threadIdx.x;
   const int c = tid & (C - 1);
   const int r = tid / C;
-  const int cbk = a.outer_x ? (int)blockIdx.y : (int)blockIdx.x;  // column block
+  // column block and batched prepare
+  int cbk = a.outer_x ? (int)blockIdx.y : (int)blockIdx.x;
+  int bz = (int)blockIdx.z;
+  if (a.nxb > 0) {
+    bz = cbk / a.nxb;
+    cbk -= bz * a.nxb;
+  }
   const int col = cbk * C + c;
   const bool okc = col < a.W;
   const CT zero = mk<TC>((TC)0, (TC)0);
@@ -185,9 +194,9 @@ __global__ __launch_bounds__((col_threads<C, R1>())) void k_colpass(const C2<TI>
   }
   const int oh = o / a.od, ol = o - oh * a.od;
   // per-block bases; every later offset is 32-bit
-  const C2<TI>* bin = in + (size_t)blockIdx.z * a.in_bs + (size_t)oh * a.in_os_hi + (size_t)ol * a.in_os_lo +
+  const C2<TI>* bin = in + (size_t)bz * a.in_bs + (size_t)oh * a.in_os_hi + (size_t)ol * a.in_os_lo +
                       (size_t)cbk * a.in_cbs;
-  C2<TO>* bout = out + (size_t)blockIdx.z * a.out_bs + (size_t)oh * a.out_os_hi + (size_t)ol * a.out_os_lo +
+  C2<TO>* bout = out + (size_t)bz * a.out_bs + (size_t)oh * a.out_os_hi + (size_t)ol * a.out_os_lo +
                  (size_t)cbk * a.out_cbs;
   const uint32_t cin = (uint32_t)c * SI, cout = (uint32_t)c * SO;
 
