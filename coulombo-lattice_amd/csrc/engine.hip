@@ -695,8 +695,10 @@ void prepare_fast(clat_engine* e, const double2* sites_in, double2* sites_out) {
       a.in_cbs = a.out_cbs = bcs;
       a.gxm = 1; a.gym = 1;
       if (lk) batch(a, (long long)e->b_set, (long long)e->b_set);
+      // algorithmic bytes: B read and written per prepare, the kernel-table
+      // octant once per launch (the batched prepares share it)
       launch2<TB, TCZ, TB, PM_CONV, false>(e, e->czc, a, 2 * Yi, 2, e->B, e->B, K_ZCONV,
-                                           (sb * 8 * N + (double)sizeof(TCZ) * (double)e->NG()) * nb);
+                                           sb * 8 * N * nb + (double)sizeof(TCZ) * (double)e->NG());
     }
     {
       P2Args a = base_args(e);
