@@ -529,7 +529,8 @@ void launch_line(clat_engine* e, const void* in, void* out, long long bs, int ki
     a.site_in2[b] = MODE == PM_FWD && b < e->cur_nb ? e->b_in2[b] : nullptr;
     a.site_out2[b] = MODE == PM_INV && b < e->cur_nb ? e->b_out2[b] : nullptr;
   }
-  if ((size_t)e->a_set > (size_t)INT32_MAX) throw host::Error(CLAT_EUNSUP, "grid too large for the line kernels");
+  if ((double)e->a_set * (double)std::max(e->s_a, e->s_d) >= (double)COL_RANGE - 4096.0)
+    throw host::Error(CLAT_EUNSUP, "grid too large for the line kernels");
   const int W = 2 * e->Xi;
   a.Yl = e->Yi;
   a.lz = e->Yi * W;

@@ -35,7 +35,7 @@ struct LineArgs {
   // dense side (A forward, D inverse; line id = z Yl + y, 2X frequencies f per
   // line): element z lz + y ly + f in plain rows (lgsw = 0), or in strips of
   // sw = 2^lgsw frequencies, z lz + y ly + (f / sw) strip + f % sw
-  // (offsets within one prepare's dense side stay below 2^31)
+  // (byte offsets within one prepare's dense side stay below COL_RANGE)
   int Yl, lgsw;
   int lz, ly, strip;
   long long in_bs, out_bs;          // element stride between batched prepares on the dense side
@@ -150,11 +150,14 @@ __global__ __launch_bounds__((line_threads<C, R1, R2, NSUB>())) void k_linepass(
 #pragma unroll
       for (int m = 0; m < R1; ++m) w[m] = lds[xi(c2, s2, k2, m)];
       rdft<R1, false, TC>(w);
-      C2<TO>* o = out + (size_t)blockIdx.y * a.out_bs + line_base(a, a.lines[b0 + c2]);
+      // block-uniform base of the prepare, 32-bit byte offsets per lane
+      C2<TO>* ob = out + (size_t)blockIdx.y * a.out_bs;
+      const int lb = line_base(a, a.lines[b0 + c2]);
       int off[R1];
       freq_offs<R1, R2 * NSUB>(a, k2 * NSUB + s2, off);
 #pragma unroll
-      for (int k1 = 0; k1 < R1; ++k1) o[off[k1]] = cvt<TC, TO>(w[k1]);
+      for (int k1 = 0; k1 < R1; ++k1)
+        stg_buf(ob, (uint32_t)(lb + off[k1]) * (uint32_t)sizeof(C2<TO>), cvt<TC, TO>(w[k1]));
     }
   } else {
     // ---------------------------------------------------------- P2 <- D
@@ -162,11 +165,13 @@ __global__ __launch_bounds__((line_threads<C, R1, R2, NSUB>())) void k_linepass(
     // costs more than their loads)
     if (ok2) {
       CT w[R1];
-      const C2<TI>* ip = in + (size_t)blockIdx.y * a.in_bs + line_base(a, a.lines[b0 + c2]);
+      const C2<TI>* ib = in + (size_t)blockIdx.y * a.in_bs;
+      const int lb = line_base(a, a.lines[b0 + c2]);
       int off[R1];
       freq_offs<R1, R2 * NSUB>(a, k2 * NSUB + s2, off);
 #pragma unroll
-      for (int k1 = 0; k1 < R1; ++k1) w[k1] = cvt<TI, TC>(ldg_stream(ip + off[k1]));
+      for (int k1 = 0; k1 < R1; ++k1)
+        w[k1] = cvt<TI, TC>(ldg_stream_off(ib, (uint32_t)(lb + off[k1]) * (uint32_t)sizeof(C2<TI>)));
       rdft<R1, true, TC>(w);
       const uint32_t tstep = (uint32_t)(2 * k2 + s2) * SC;
       // twiddles W_2L^{m q}, q = 2 k2 + s2.  FP64: the powers of two m come
