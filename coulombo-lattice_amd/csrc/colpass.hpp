@@ -4,7 +4,9 @@
 //   * keeps one sub-line in LDS at a time (the parities run one after the
 //     other), halving the LDS per block;
 //   * addresses LDS with compile-time offsets from one per-thread base, and
-//     global memory with 32-bit offsets from a per-block base pointer;
+//     global memory through buffer resources: a per-block base in scalar
+//     registers and 32-bit lane offsets (masked rows take an out-of-range
+//     offset instead of a branch);
 //   * folds the half-shift phase h[n] = W_2L^n = W_2L^n1 W_2R2^n2 into the
 //     pass twiddle (W_2L^{n1 (2 k2 + s)}, one table of 2L roots) and a
 //     compile-time pre/post rotation by W_2R2^n2.
@@ -24,7 +26,8 @@ struct ColArgs2 {
   //   (o / od) * os_hi + (o % od) * os_lo
   int od;
   long long in_os_hi, in_os_lo, out_os_hi, out_os_lo;
-  long long in_bs, out_bs;        // element stride between batched prepares (blockIdx.z)
+  long long in_bs, out_bs;        // element stride between batched prepares (blockIdx.z, or
+                                  // blockIdx.x / nxb when folded)
   long long in_cbs, out_cbs;      // element stride between blocks of C columns (blockIdx.x): C for
                                   // row-major planes, C x rows for planes tiled in C-column strips
   // rows: n-domain row n1 + R1 n2 -> byte offset n2 * is_hi + n1 * is_lo;
