@@ -38,7 +38,8 @@ SYMBOLS = [
     "clat_set_interaction_simple", "clat_set_interaction_thomas_fermi", "clat_set_interaction_table",
     "clat_set_input", "clat_prepare", "clat_calculate", "clat_get_potential", "clat_set_atoms",
     "clat_add_state", "clat_clear_states", "clat_compute_integrals", "clat_compute_potential_at_atoms",
-    "clat_last_stage_ms", "clat_set_kernel_timing", "clat_kernel_kind_count", "clat_kernel_kind_name",
+    "clat_last_stage_ms", "clat_set_kernel_timing", "clat_set_kernel_timing_kinds", "clat_kernel_kind_count",
+    "clat_kernel_kind_name",
     "clat_kernel_stats", "clat_reset_kernel_stats", "clat_engine_info",
 ]
 
@@ -85,6 +86,7 @@ def lib() -> ctypes.CDLL:
     L.clat_compute_potential_at_atoms.argtypes = [vp, ctypes.c_int, _dp]
     L.clat_last_stage_ms.argtypes = [vp, _dp]
     L.clat_set_kernel_timing.argtypes = [vp, ctypes.c_int]
+    L.clat_set_kernel_timing_kinds.argtypes = [vp, ctypes.c_uint32]
     L.clat_kernel_kind_count.restype = ctypes.c_int
     L.clat_kernel_kind_name.argtypes = [ctypes.c_int]
     L.clat_kernel_kind_name.restype = ctypes.c_char_p
@@ -321,6 +323,16 @@ class Engine:
 
     def set_kernel_timing(self, on: bool) -> None:
         _check(lib().clat_set_kernel_timing(self.handle, 1 if on else 0))
+
+    def set_kernel_timing_kinds(self, names) -> None:
+        """Time only the kernel kinds named (the names of kernel_kinds())."""
+        known = kernel_kinds()
+        mask = 0
+        for n in names:
+            if n not in known:
+                raise ValueError("unknown kernel kind %r" % n)
+            mask |= 1 << known.index(n)
+        _check(lib().clat_set_kernel_timing_kinds(self.handle, mask))
 
     def reset_kernel_stats(self) -> None:
         _check(lib().clat_reset_kernel_stats(self.handle))

@@ -287,7 +287,7 @@ struct clat_engine {
   // timing
   hipEvent_t stage_ev[6] = {};
   double stage_ms[5] = {0, 0, 0, 0, 0};
-  bool ktiming = false;
+  uint32_t kmask = 0;  // kinds whose launches are timed (bit = kernel kind)
   std::vector<hipEvent_t> kev;
   std::vector<std::pair<int, double>> kpending;
   size_t kused = 0;
@@ -314,13 +314,14 @@ struct clat_engine {
   size_t NU() const { return (size_t)X * Y * Z; }
   bool shared() const { return sched == CLAT_SCHED_SHARED; }
 
-  void kbegin() {
-    if (!ktiming) return;
+  bool ktimed(int kind) const { return (kmask >> kind) & 1u; }
+  void kbegin(int kind) {
+    if (!ktimed(kind)) return;
     ensure_events(kused + 2);
     HIPX(hipEventRecord(kev[kused], stream));
   }
   void kend(int kind, double bytes) {
-    if (!ktiming) return;
+    if (!ktimed(kind)) return;
     HIPX(hipEventRecord(kev[kused + 1], stream));
     kpending.push_back({kind, bytes});
     kused += 2;
@@ -333,7 +334,7 @@ struct clat_engine {
     }
   }
   void kcollect() {
-    if (!ktiming) return;
+    if (kpending.empty()) return;
     HIPX(hipStreamSynchronize(stream));
     // (the table-build events of settle_table are complete from here on)
     for (size_t i = 0; i < kpending.size(); ++i) {
@@ -451,7 +452,7 @@ bool launch_col(clat_engine* e, const PassCfg& pc, const P2Args& a, int n_outer,
   L.T2L = sizeof(TC) == 8 ? (const void*)e->dtw[axis] : (const void*)e->dtw32[axis];
   L.G = gc_of<TC>(e);
   constexpr int combo = combo_of<TI, TC, TO>();
-  e->kbegin();
+  e->kbegin(kind);
   hipError_t err = hipErrorInvalidValue;
   switch (pc.R1 * 64 + pc.R2) {
 #define CLAT_C2_CASE(ra, rb) \
@@ -495,7 +496,7 @@ void launch2(clat_engine* e, const PassCfg& pc, P2Args a, int n_outer, int axis,
   L.hz = h_of<TC>(e, 2);
   L.G = gc_of<TC>(e);
   constexpr int combo = combo_of<TI, TC, TO>();
-  e->kbegin();
+  e->kbegin(kind);
   hipError_t err;
   switch (pc.R1 * 64 + pc.R2) {
 #define CLAT_P2_CASE(ra, rb) \
@@ -557,7 +558,7 @@ void launch_line(clat_engine* e, const void* in, void* out, long long bs, int ki
   if (lines <= 0) throw host::Error(CLAT_EDEVICE, "line kernel missing for the x radix pair");
   if (e->n_live == 0) return;
   L.blocks = (unsigned)((e->n_live + lines - 1) / lines);
-  e->kbegin();
+  e->kbegin(kind);
   hipError_t err = hipErrorInvalidValue;
   switch (R1 * 64 + R2) {
 #define CLAT_L2_CASE(ra, rb) case ra * 64 + rb: err = clat_l2_##ra##x##rb(combo, MODE, L); break;
@@ -765,7 +766,7 @@ void prepare_generic(clat_engine* e) {
     const int C = tile_lines(e, nsub, Xi);
     const size_t lds = (size_t)nsub * C * Xi * sizeof(CT);
     dim3 grid((nlines + C - 1) / C);
-    e->kbegin();
+    e->kbegin(K_XFWD);
     hipLaunchKernelGGL((k_xfwd<T, EMAX>), grid, dim3(NT), lds, e->stream, F, out, e->px, twx, hx, nlines, C,
                        log2i(C), nsub, kx, ky, kz, hy, hz, Yi);
     LAUNCH_CHECK();
@@ -775,7 +776,7 @@ void prepare_generic(clat_engine* e) {
     const int C = tile_lines(e, nsub, Xi);
     const size_t lds = (size_t)nsub * C * Xi * sizeof(CT);
     dim3 grid((nlines + C - 1) / C);
-    e->kbegin();
+    e->kbegin(K_XINV);
     hipLaunchKernelGGL((k_xinv<T, EMAX>), grid, dim3(NT), lds, e->stream, in, V, e->px, twx, hx, nlines, C,
                        log2i(C), nsub, kx, ky, kz, hy, hz, Yi, acc);
     LAUNCH_CHECK();
@@ -789,7 +790,8 @@ void prepare_generic(clat_engine* e) {
     a.lgC = log2i(a.C);
     const size_t lds = (size_t)a.nsub * a.C * L * sizeof(CT);
     dim3 grid((a.W + a.C - 1) / a.C, n_outer);
-    e->kbegin();
+    const int kind0 = mode == COL_FWD ? K_YFWD : (mode == COL_CONV ? K_ZCONV : K_YINV);
+    e->kbegin(kind0);
     int kind;
     double bytes;
     if (mode == COL_FWD) {
@@ -912,7 +914,7 @@ void prepare_sites(clat_engine* e, const double2* rho_sites, double2* v_sites) {
   }
   ensure_dense(e);
   zero_grid(e, e->F, e->N(), e->s_in);
-  e->kbegin();
+  e->kbegin(K_SCATTER);
   if (e->s_in == 8)
     hipLaunchKernelGGL(k_scatter<float>, dim3(grid1d(e->n_sites)), dim3(256), 0, e->stream, (float2*)e->F,
                        e->d_site_cell, rho_sites, e->n_sites);
@@ -923,7 +925,7 @@ void prepare_sites(clat_engine* e, const double2* rho_sites, double2* v_sites) {
   e->kend(K_SCATTER, (double)e->n_sites * (20.0 + (double)e->s_in));
   if (e->rprec == CLAT_PREC_FP32) prepare_generic<float, 32>(e);
   else prepare_generic<double, 16>(e);
-  e->kbegin();
+  e->kbegin(K_GATHER);
   if (e->s_v == 8)
     hipLaunchKernelGGL(k_gather<float>, dim3(grid1d(e->n_sites)), dim3(256), 0, e->stream, (const float2*)e->V,
                        e->d_site_cell, v_sites, e->n_sites);
@@ -943,7 +945,7 @@ void dct_axis(clat_engine* e, int axis, double scale) {
   const int L2 = p.L;
   const int C = std::min(64, floor_pow2(std::max(1, 4096 / L2)));
   const size_t lds = (size_t)C * L2 * sizeof(double2);
-  e->kbegin();
+  e->kbegin(K_DCT);
   if (axis == 0) {
     const int ncols = Y1 * Z1;
     dim3 grid((ncols + 2 * C - 1) / (2 * C));
@@ -1064,7 +1066,7 @@ void build_products(clat_engine* e) {
   int S = 64;
   while (S > 1 && (size_t)(S + 1) * per_site > 23040) S /= 2;
   if (S == 1 && 3 * per_site <= 45056) S = 2;
-  e->kbegin();
+  e->kbegin(K_PRODUCTS);
   if ((size_t)(S + 1) * per_site <= 65536) {
     const unsigned nb = (unsigned)((e->n_sites + S - 1) / S);
     hipLaunchKernelGGL(k_products, dim3(nb), dim3(NT), (size_t)(S + 1) * per_site, e->stream, e->coeffs,
@@ -1713,7 +1715,7 @@ int clat_compute_integrals(clat_engine* e, const int32_t* quads, int64_t n, int 
     }
     std::vector<double> ov(n_out, 0.0);
     if (nL > 0 && e->n_sites > 0) {
-      e->kbegin();
+      e->kbegin(K_OVERLAP);
       const dim3 og(tl, tr, (unsigned)n_chunks);
       hipLaunchKernelGGL(k_overlap_partial<TX>, og, dim3(TX * TX), 0, e->stream, e->Vat, e->rho, nL, P, e->n_sites,
                          chunk, e->part);
@@ -1796,7 +1798,13 @@ int clat_last_stage_ms(clat_engine* e, double ms[5]) {
 
 int clat_set_kernel_timing(clat_engine* e, int enable) {
   if (!e) return set_err(CLAT_EINVAL, "invalid arguments");
-  e->ktiming = enable != 0;
+  e->kmask = enable ? ~0u : 0u;
+  return CLAT_OK;
+}
+
+int clat_set_kernel_timing_kinds(clat_engine* e, uint32_t kinds) {
+  if (!e) return set_err(CLAT_EINVAL, "invalid arguments");
+  e->kmask = kinds;
   return CLAT_OK;
 }
 

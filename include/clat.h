@@ -184,6 +184,12 @@ int clat_last_stage_ms(clat_engine* e, double ms[5]);
  * engine stream and timed per kernel kind. */
 int clat_set_kernel_timing(clat_engine* e, int enable);
 
+/* The same for a subset of kernel kinds: bit k of `kinds` times kind k (0
+ * times nothing).  A measurement hook with no counterpart in the reference:
+ * the events cost a few microseconds per launch, so a run that reports one
+ * kernel's duration times only that kernel. */
+int clat_set_kernel_timing_kinds(clat_engine* e, uint32_t kinds);
+
 /* Per-kind totals since the last reset: launches, summed milliseconds and the
  * algorithmic bytes those launches must move (one read of each input element
  * and one write of each output element).  kind < clat_kernel_kind_count(). */
