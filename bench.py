@@ -88,6 +88,68 @@ def traffic_of(kind):
         return None
 
 
+def multi_system(args, clat, zb, np):
+    """Several zincblende systems of uneven size, one engine each (FP32X): the
+    step is every system's interaction build and integrals, run one after
+    another and then concurrently (one host thread and one stream per engine).
+    Four orbitals per atom keep the host-side generation of the 8M-atom
+    coefficients short; only the pair-density kernel reads them."""
+    sizes = [int(v) for v in args.systems.split(",")]
+    nh, ne = args.states
+    n_orb = 4
+    systems = []
+    for k, n in enumerate(sizes):
+        t0 = time.time()
+        w = zb.workload(n, n, n, nh, ne, config=4, sigma_frac=0.15, n_orb=n_orb)
+        X, Y, Z = w["dims"]
+        eng = clat.Engine((X, Y, Z), clat.PREC_FP32X, clat.SCHED_SHARED)
+        eng.set_atoms(w["cells"], n_orb)
+        for st in w["states"]:
+            eng.add_state(st)
+        quads = clat.pattern_enumerate(args.integrals, nh + ne, nh)
+        inter = clat.InteractionSimple(w["step"], 0.0, 10.89)
+        systems.append((eng, inter, quads, (X, Y, Z), len(w["cells"])))
+        del w
+        log("system %d: %d^3 cells, grid %dx%dx%d, %d atoms, %d integrals (%.1f s)"
+            % (k, n, X, Y, Z, systems[-1][4], len(quads), time.time() - t0))
+
+    def job(sys_):
+        eng, inter, quads = sys_[0], sys_[1], sys_[2]
+        eng.initialize(inter)
+        return eng.compute_integrals(quads)
+
+    for _ in range(max(1, args.warmup)):
+        for sy in systems:
+            job(sy)
+    t = time.perf_counter()
+    for _ in range(args.steps):
+        seq = [job(sy) for sy in systems]
+    t_seq = (time.perf_counter() - t) / args.steps
+    log("one after another: %.2f ms per step" % (1e3 * t_seq))
+    t = time.perf_counter()
+    for _ in range(args.steps):
+        con = clat.run_concurrent([(lambda sy=sy: job(sy)) for sy in systems])
+    t_con = (time.perf_counter() - t) / args.steps
+    log("concurrent: %.2f ms per step" % (1e3 * t_con))
+    same = all(np.array_equal(a, b) for a, b in zip(seq, con))
+    n_int = sum(len(sy[2]) for sy in systems)
+    out = {
+        "metric": "integrals/s, multi-system batch (one engine per system, concurrent)",
+        "value": n_int / t_con, "unit": "integrals/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": 1e3 * t_con, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "c64 storage, f64 arithmetic and accumulation",
+        "data": "synthetic (seeded zincblende, %d orbitals per atom)" % n_orb,
+        "config": {"workload": "zincblende boxes of %s cells, %d states each, %s"
+                               % ("/".join("%d^3" % n for n in sizes), nh + ne, args.integrals),
+                   "grids": ["%dx%dx%d" % sy[3] for sy in systems], "atoms": [sy[4] for sy in systems]},
+        "sequential_ms_per_step": 1e3 * t_seq, "concurrent_speedup": t_seq / t_con,
+        "concurrent_equals_sequential": bool(same),
+    }
+    for sy in systems:
+        sy[0].close()
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -100,6 +162,9 @@ def main():
     ap.add_argument("--schedule", choices=["shared", "rounds"], default="shared")
     ap.add_argument("--integrals", default="eeee,hhhh,ehhe,eheh",
                     help="integral pattern list (ijkl with --cells 32 32 32 --states 16 16: the batched form)")
+    ap.add_argument("--systems", default="",
+                    help="comma-separated cubic box sizes in cells (e.g. 16,24,30,40,48,100): the multi-system "
+                         "workload, one engine per system, timed one after another and concurrently")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a plain device-to-device copy of MIB MiB (counter calibration)")
     args = ap.parse_args()
@@ -121,6 +186,10 @@ def main():
     import numpy as np
     import clat
     import zincblende as zb
+
+    if args.systems:
+        multi_system(args, clat, zb, np)
+        return
 
     nx, ny, nz = args.cells
     nh, ne = args.states

@@ -353,3 +353,19 @@ class Engine:
 
     def close(self):
         self._h.close()
+
+
+def run_concurrent(jobs) -> list:
+    """Run several jobs at once, one host thread each, and return their results
+    in order.  Each job is a callable without arguments that uses engines of
+    its own (an engine handle is single-threaded).  Every engine launches on
+    its own stream and the library holds no shared state, so the GPU overlaps
+    the engines' kernels: the passes of small grids fill the device while a
+    large grid streams.  ctypes releases the interpreter lock during each
+    library call."""
+    from concurrent.futures import ThreadPoolExecutor
+    lib()  # load the library before the threads start
+    jobs = list(jobs)
+    with ThreadPoolExecutor(max_workers=max(1, len(jobs))) as ex:
+        futures = [ex.submit(j) for j in jobs]
+        return [f.result() for f in futures]

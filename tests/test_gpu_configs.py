@@ -96,3 +96,21 @@ def test_multi_system_sizes(cells):
         Gh = O.ghat(O.gtable(X, Y, Z, w["step"], 0.0, 10.89))
         want = O.compute_integrals([tuple(q) for q in quads], 2, rho, w["cells"], (X, Y, Z), Gh)
         assert _rel_err(vals[clat.PREC_FP64], want) < 1e-10
+
+
+def test_multi_system_concurrent():
+    """Engines of different grid sizes run from separate host threads (one
+    stream each) give exactly the values they give one after another."""
+    print("case config multi-system concurrent 16^3 + 24^3 + 30^3", flush=True)
+    jobs = []
+    for n in (16, 24, 30):
+        w = zb.workload(n, n, n, 2, 2, config=4, sigma_frac=0.15, n_orb=4)
+        eng = _engine(w, clat.PREC_FP32X, 4)
+        jobs.append((eng, clat.pattern_enumerate("****", 4, 2)))
+    seq = [e.compute_integrals(q) for e, q in jobs]
+    for _ in range(3):
+        con = clat.run_concurrent([(lambda e=e, q=q: e.compute_integrals(q)) for e, q in jobs])
+        for a, b in zip(seq, con):
+            assert np.array_equal(a, b)
+    for e, _ in jobs:
+        e.close()
