@@ -137,7 +137,8 @@ def multi_system(args, clat, zb, np):
         "metric": "integrals/s, multi-system batch (one engine per system, concurrent)",
         "value": n_int / t_con, "unit": "integrals/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": 1e3 * t_con, "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": "c64 (FP32 FFT passes; FP64 last pass, kernel table, pair densities and overlap sums)",
+        "vs_baseline": None,
+        "dtype": "c64 (FP32 FFT passes; FP64 last pass, kernel table, pair densities and overlap sums)",
         "data": "synthetic (seeded zincblende, %d orbitals per atom)" % n_orb,
         "config": {"workload": "zincblende boxes of %s cells, %d states each, %s"
                                % ("/".join("%d^3" % n for n in sizes), nh + ne, args.integrals),
