@@ -75,10 +75,13 @@ def cpu_baseline(w, quads, budget_s=12.0):
                       "run" % (n_prep, len(lefts), n_ovl, sampled, len(quads))}
 
 
-def traffic_of(kind):
+def traffic_of(kind, default_workload):
     """HBM bytes per launch of a pass kind from the committed rocprofv3 counter
-    summary (FETCH_SIZE + WRITE_SIZE, scaled by the plain-copy calibration),
-    or None when no summary exists for it."""
+    summary (FETCH_SIZE + WRITE_SIZE, scaled by the plain-copy calibration) of
+    the default workload (1M atoms, 8 states), or None for another workload or
+    when no summary exists for the kind."""
+    if not default_workload:
+        return None
     path = os.path.join(ROOT, "profiles", "traffic_1M_atoms.json")
     try:
         with open(path) as f:
@@ -269,6 +272,8 @@ def main():
         eng.close()
 
     head = res["fp32x"]
+    default_workload = (list(args.cells) == [64, 64, 32] and list(args.states) == [4, 4]
+                        and args.integrals == "eeee,hhhh,ehhe,eheh" and args.schedule == "shared")
     ms = 1e3 * head["total"] / args.steps
     value = len(quads) / (head["total"] / args.steps)
     # roofline of the dominant kernel, from HIP events around each of its
@@ -288,7 +293,7 @@ def main():
                                % (nx, ny, nz, len(w["cells"]), X, Y, Z, nh + ne, args.integrals),
                    "integrals": len(quads), "schedule": args.schedule},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                     "frac": achieved / PEAK_HBM_GBS, "traffic": traffic_of(dom), "avg_launch_ms": avg_ms,
+                     "frac": achieved / PEAK_HBM_GBS, "traffic": traffic_of(dom, default_workload), "avg_launch_ms": avg_ms,
                      "algorithmic_bytes_per_launch": byts / max(1, n_l)},
         "kernel_ms_per_step": {k: v[1] for k, v in head["breakdown"].items() if v[0]},
         "fft_pass_gbs": {k: (v[2] / max(1e-12, v[1] * 1e-3)) / 1e9 for k, v in head["breakdown"].items()
