@@ -293,7 +293,8 @@ def main():
                                % (nx, ny, nz, len(w["cells"]), X, Y, Z, nh + ne, args.integrals),
                    "integrals": len(quads), "schedule": args.schedule},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                     "frac": achieved / PEAK_HBM_GBS, "traffic": traffic_of(dom, default_workload), "avg_launch_ms": avg_ms,
+                     "frac": achieved / PEAK_HBM_GBS, "traffic": traffic_of(dom, default_workload),
+                     "avg_launch_ms": avg_ms,
                      "algorithmic_bytes_per_launch": byts / max(1, n_l)},
         "kernel_ms_per_step": {k: v[1] for k, v in head["breakdown"].items() if v[0]},
         "fft_pass_gbs": {k: (v[2] / max(1e-12, v[1] * 1e-3)) / 1e9 for k, v in head["breakdown"].items()
