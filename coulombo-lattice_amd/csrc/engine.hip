@@ -269,6 +269,7 @@ struct clat_engine {
   const double2* b_in2[LINE_NB] = {};  // second real density packed into the prepare (or null)
   double2* b_out2[LINE_NB] = {};
   bool pack = true;                     // pairs of real (diagonal) densities share a prepare
+  int ovl_mode = 0;                     // overlap kernel: 0 by size, 1 vector FMA, 2 FP64 MFMA
   bool table_pending = false;           // stage_ms[0] of the last table build not read yet
   size_t a_set = 0, b_set = 0, d_set = 0;  // elements of A, B, D per prepare
   bool xline = false;                // x passes on the line kernel, y passes skip empty lines
@@ -1348,6 +1349,8 @@ int clat_create(int X, int Y, int Z, int precision, int schedule, clat_engine** 
       while (e->nb_max < LINE_NB && (size_t)(2 * e->nb_max) * set_bytes <= ((size_t)16 << 30)) e->nb_max *= 2;
     if (const char* g = std::getenv("CLAT_BATCH")) e->nb_max = std::max(1, std::min(e->nb_max, std::atoi(g)));
     if (const char* g = std::getenv("CLAT_PACK")) e->pack = std::atoi(g) != 0;
+    if (const char* g = std::getenv("CLAT_OVL"))
+      e->ovl_mode = std::strcmp(g, "valu") == 0 ? 1 : std::strcmp(g, "mfma") == 0 ? 2 : 0;
     e->a_bytes = e->nb_max * e->a_set * e->s_a;
     e->A = e->dalloc(e->a_bytes);
     if (e->shared()) {
@@ -1692,15 +1695,18 @@ int clat_compute_integrals(clat_engine* e, const int32_t* quads, int64_t n, int 
     }
     HIPX(hipEventRecord(e->stage_ev[3], e->stream));
     // overlaps of every (slot, product) pair
-    // tile edge 3 TX with TX x TX threads (TX = 16 measured fastest for tens of products)
+    // tile edge 3 TX with TX x TX threads (TX = 16 measured fastest for tens of products);
+    // 64 x 64 tiles on FP64 MFMA when both sides fill them
     constexpr int TX = 16;
-    const int OV_T = 3 * TX;
+    const bool mfma = e->ovl_mode == 2 || (e->ovl_mode == 0 && nL >= OVM_MIN && P >= OVM_MIN);
+    const int OV_T = mfma ? OVM_T : 3 * TX;
+    const int ovs = mfma ? OVM_S : OV_S;
     const int tl = (nL + OV_T - 1) / OV_T, tr = (P + OV_T - 1) / OV_T;
     int64_t n_chunks = std::max<int64_t>(1, 2048 / std::max(1, tl * tr));
-    n_chunks = std::min<int64_t>(n_chunks, (e->n_sites + OV_S - 1) / OV_S);
+    n_chunks = std::min<int64_t>(n_chunks, (e->n_sites + ovs - 1) / ovs);
     n_chunks = std::max<int64_t>(1, n_chunks);
     int64_t chunk = (e->n_sites + n_chunks - 1) / n_chunks;
-    chunk = std::max<int64_t>(OV_S, (chunk + OV_S - 1) / OV_S * OV_S);
+    chunk = std::max<int64_t>(ovs, (chunk + ovs - 1) / ovs * ovs);
     n_chunks = std::max<int64_t>(1, (e->n_sites + chunk - 1) / chunk);
     const size_t n_out = (size_t)std::max(1, nL) * P * 4;
     if (e->part_cap < n_out * n_chunks) {
@@ -1717,8 +1723,12 @@ int clat_compute_integrals(clat_engine* e, const int32_t* quads, int64_t n, int 
     if (nL > 0 && e->n_sites > 0) {
       e->kbegin(K_OVERLAP);
       const dim3 og(tl, tr, (unsigned)n_chunks);
-      hipLaunchKernelGGL(k_overlap_partial<TX>, og, dim3(TX * TX), 0, e->stream, e->Vat, e->rho, nL, P, e->n_sites,
-                         chunk, e->part);
+      if (mfma)
+        hipLaunchKernelGGL(k_overlap_partial_mfma, og, dim3(256), 0, e->stream, e->Vat, e->rho, nL, P, e->n_sites,
+                           chunk, e->part);
+      else
+        hipLaunchKernelGGL(k_overlap_partial<TX>, og, dim3(TX * TX), 0, e->stream, e->Vat, e->rho, nL, P,
+                           e->n_sites, chunk, e->part);
       LAUNCH_CHECK();
       hipLaunchKernelGGL(k_overlap_reduce, dim3((unsigned)((n_out + 63) / 64)), dim3(1024), 0, e->stream, e->part,
                          n_chunks, (int64_t)n_out, e->ovl);

@@ -638,6 +638,94 @@ __global__ __launch_bounds__(TX * TX) void k_overlap_partial(const double2* __re
     }
 }
 
+// Stage 1 on FP64 MFMA, for many slots and products: the four sums of a
+// 64 x 64 (slot, product) tile are four real matrix products over the sites,
+//   A = Vr Pr^T, B = Vi Pi^T, C = Vr Pi^T, D = Vi Pr^T,
+// each a 4 x 4 grid of 16 x 16 v_mfma_f64_16x16x4f64 tiles.  Wave w owns
+// slots 16w..16w+15 against all 64 products: 16 independent accumulator
+// chains (4 product tiles x 4 sums).  Operand lanes: A[i][k] in lane
+// i + 16k (slot i, site k), B[k][j] in lane j + 16k (site k, product j);
+// result D[i][j] in lane j + 16 (i % 4), element i / 4.
+// Same chunking and partial layout as k_overlap_partial.
+constexpr int OVM_T = 64;   // tile edge (slots and products)
+constexpr int OVM_S = 16;   // sites staged per step
+constexpr int OVM_MIN = 96; // both sides at least this many: the MFMA tiles are used
+
+__global__ __launch_bounds__(256) void k_overlap_partial_mfma(const double2* __restrict__ Vat,
+                                                              const double2* __restrict__ rho,
+                                                              int n_left, int n_right, int64_t n_sites,
+                                                              int64_t chunk, double* __restrict__ part) {
+  typedef double d4 __attribute__((ext_vector_type(4)));
+  constexpr int P = OVM_S + 1;  // LDS pitch in double2
+  __shared__ double2 sv[OVM_T * P];
+  __shared__ double2 sr[OVM_T * P];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int l0 = blockIdx.x * OVM_T, r0 = blockIdx.y * OVM_T;
+  const int64_t s0 = (int64_t)blockIdx.z * chunk;
+  const int64_t s1 = min(n_sites, s0 + chunk);
+  d4 acc[4][4];
+#pragma unroll
+  for (int t = 0; t < 4; ++t)
+#pragma unroll
+    for (int q = 0; q < 4; ++q) acc[t][q] = d4{0.0, 0.0, 0.0, 0.0};
+  // staging: 64 rows x 16 sites per table, 4 elements per thread and table;
+  // 16 consecutive threads read one row's 256 contiguous bytes
+  constexpr int NE = OVM_T * OVM_S / 256;
+  double2 bv[NE], br[NE];
+  auto fetch = [&](int64_t sb) {
+#pragma unroll
+    for (int j = 0; j < NE; ++j) {
+      const int e = threadIdx.x + j * 256;
+      const int row = e >> 4, col = e & 15;
+      const int64_t site = sb + col;
+      const bool ok = site < s1;
+      bv[j] = (ok && l0 + row < n_left) ? nt_load(Vat + (size_t)(l0 + row) * n_sites + site) : make_double2(0.0, 0.0);
+      br[j] = (ok && r0 + row < n_right) ? nt_load(rho + (size_t)(r0 + row) * n_sites + site) : make_double2(0.0, 0.0);
+    }
+  };
+  const int fi = lane & 15, fk = lane >> 4;  // operand row/column and site within a 4-site step
+  if (s0 < s1) fetch(s0);
+  for (int64_t sb = s0; sb < s1; sb += OVM_S) {
+#pragma unroll
+    for (int j = 0; j < NE; ++j) {
+      const int e = threadIdx.x + j * 256;
+      const int row = e >> 4, col = e & 15;
+      sv[row * P + col] = bv[j];
+      sr[row * P + col] = br[j];
+    }
+    __syncthreads();
+    if (sb + OVM_S < s1) fetch(sb + OVM_S);
+#pragma unroll
+    for (int k = 0; k < OVM_S; k += 4) {
+      const double2 a = sv[(16 * w + fi) * P + k + fk];
+      double2 b[4];
+#pragma unroll
+      for (int t = 0; t < 4; ++t) b[t] = sr[(16 * t + fi) * P + k + fk];
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        acc[t][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a.x, b[t].x, acc[t][0], 0, 0, 0);
+        acc[t][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a.y, b[t].y, acc[t][1], 0, 0, 0);
+        acc[t][2] = __builtin_amdgcn_mfma_f64_16x16x4f64(a.x, b[t].y, acc[t][2], 0, 0, 0);
+        acc[t][3] = __builtin_amdgcn_mfma_f64_16x16x4f64(a.y, b[t].x, acc[t][3], 0, 0, 0);
+      }
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int t = 0; t < 4; ++t) {
+    const int r = r0 + 16 * t + fi;
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int l = l0 + 16 * w + 4 * v + fk;
+      if (l < n_left && r < n_right) {
+        double2* d = reinterpret_cast<double2*>(part + (((size_t)blockIdx.z * n_left + l) * n_right + r) * 4);
+        d[0] = make_double2(acc[t][0][v], acc[t][1][v]);
+        d[1] = make_double2(acc[t][2][v], acc[t][3][v]);
+      }
+    }
+  }
+}
+
 // Stage 2: compensated sum of the chunk partials.  A block owns 64 outputs;
 // its 16 row groups of lanes each run a Neumaier sum over a stride of the
 // chunks, then the 16 (sum, compensation) pairs are merged in LDS.

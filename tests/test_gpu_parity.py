@@ -242,6 +242,43 @@ def test_batched_prepares(prec, monkeypatch):
     scale = np.max(np.abs(want))
     err = np.abs(got - want) / np.maximum(np.abs(want), 1e-3 * scale)
     assert np.max(err) < TOL[prec], np.max(err)
+    # overlap sums on FP64 MFMA tiles (forced here; 15 slots x 15 products fill
+    # one 64 x 64 tile partially): the same sums in another order
+    monkeypatch.delenv("CLAT_PACK")
+    monkeypatch.setenv("CLAT_OVL", "mfma")
+    on_mfma = run()
+    assert _rel(on_mfma, got) < 1e-12, _rel(on_mfma, got)
+    err = np.abs(on_mfma - want) / np.maximum(np.abs(want), 1e-3 * scale)
+    assert np.max(err) < TOL[prec], np.max(err)
+
+
+@pytest.mark.parametrize("prec", [clat.PREC_FP64, clat.PREC_FP32X], ids=["fp64", "fp32x"])
+def test_overlap_tiles_by_size(prec, monkeypatch):
+    """14 states: 105 left slots and 105 products, where the overlap sums run
+    on FP64 MFMA tiles (two 64 x 64 tiles per side, the second partial); the
+    vector-FMA kernel (three 48 x 48 tiles per side) gives the same integrals
+    to FP64 rounding.  The kernel that ran is told by the overlap's staged
+    bytes, tiles x 2 x edge x sites x 16."""
+    print("case overlap_tiles", prec, flush=True)
+    w = zb.workload(4, 4, 4, 7, 7, config=0, sigma_frac=0.2, n_orb=4)
+    X, Y, Z = w["dims"]
+    quads = clat.pattern_enumerate("****", 14, 7)
+
+    def run():
+        eng = clat.Engine((X, Y, Z), prec, clat.SCHED_SHARED)
+        eng.initialize(clat.InteractionSimple(w["step"], 0.0, 10.89))
+        eng.set_atoms(w["cells"], 4)
+        for st in w["states"]:
+            eng.add_state(st)
+        eng.set_kernel_timing(True)
+        vals = eng.compute_integrals(quads)
+        return vals, eng.kernel_stats()["overlap"][2]
+
+    got, staged = run()
+    monkeypatch.setenv("CLAT_OVL", "valu")
+    ref, staged_ref = run()
+    assert staged > 0 and staged * (9 * 96) == staged_ref * (4 * 128), (staged, staged_ref)
+    assert _rel(got, ref) < 1e-12, _rel(got, ref)
 
 
 @pytest.mark.parametrize("prec", [clat.PREC_FP64, clat.PREC_FP32, clat.PREC_FP32X], ids=["fp64", "fp32", "fp32x"])
