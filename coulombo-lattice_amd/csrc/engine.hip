@@ -1637,14 +1637,18 @@ int clat_compute_integrals(clat_engine* e, const int32_t* quads, int64_t n, int 
   if (!e->initialized) return set_err(CLAT_ESTATE, "calculator is not initialized");
   if (e->n_states <= 0) return set_err(CLAT_ESTATE, "no states loaded");
   return guarded([&] {
-    std::vector<host::PlanStep> steps = host::make_plan(quads, n, e->n_states);
+    // the stale-values bookkeeping walks the steps in execution order; without
+    // it every value depends on its own step only and id order serves
+    const bool stale = (flags & CLAT_FLAG_STALE_VALUES) != 0;
+    std::vector<host::PlanStep> steps = host::make_plan(quads, n, e->n_states, stale);
     const int P = e->n_states * (e->n_states + 1) / 2;
-    // left products in execution order -> slots
+    // left products in execution order (ascending product index) -> slots
     std::vector<int> slot_of((size_t)P, -1), slots;
-    for (const auto& s : steps)
-      if (slot_of[(size_t)s.left.product] < 0) {
-        slot_of[(size_t)s.left.product] = (int)slots.size();
-        slots.push_back(s.left.product);
+    for (const auto& s : steps) slot_of[(size_t)s.left.product] = 0;
+    for (int p = 0; p < P; ++p)
+      if (slot_of[(size_t)p] == 0) {
+        slot_of[(size_t)p] = (int)slots.size();
+        slots.push_back(p);
       }
     const int nL = (int)slots.size();
     HIPX(hipEventRecord(e->stage_ev[1], e->stream));
@@ -1740,7 +1744,6 @@ int clat_compute_integrals(clat_engine* e, const int32_t* quads, int64_t n, int 
     HIPX(hipStreamSynchronize(e->stream));
     e->kcollect();
     // main loop bookkeeping (coulombo.cpp:186-209)
-    const bool stale = (flags & CLAT_FLAG_STALE_VALUES) != 0;
     int lastR = -1, lastRc = 0;
     double vre = 0.0, vim = 0.0;
     for (const auto& s : steps) {

@@ -289,7 +289,9 @@ inline PlanProduct plan_product(int iL, int iR) {
   return PlanProduct{iL * (iL - 1) / 2 + iR - 1, conj};
 }
 
-inline std::vector<PlanStep> make_plan(const int32_t* quads, int64_t n, int n_states) {
+// ordered = false: the steps in id order after the cover swap, for callers
+// that need each step's products but not the execution order
+inline std::vector<PlanStep> make_plan(const int32_t* quads, int64_t n, int n_states, bool ordered = true) {
   if (n > std::numeric_limits<int>::max()) throw Error(1, "too many integrals to compute");
   std::vector<PlanStep> steps((size_t)n);
   const int64_t P = (int64_t)n_states * (n_states + 1) / 2;
@@ -326,10 +328,27 @@ inline std::vector<PlanStep> make_plan(const int32_t* quads, int64_t n, int n_st
   for (int v : cv) in_cover[(size_t)v] = 1;
   for (PlanStep& s : steps)
     if (!in_cover[(size_t)s.left.product]) std::swap(s.left, s.right);
+  if (!ordered) return steps;
   // order (Planner.hpp:94-101): left product, right product, then the steps
   // whose two products have equal conj flags before the others
   auto rc = [](const PlanStep& s) { return (s.left.conj != s.right.conj) ? 1 : 0; };
-  if (P * P * 2 < ((int64_t)1 << 32)) {
+  const int64_t n_keys = P * P * 2;
+  if (n_keys <= std::max<int64_t>((int64_t)1 << 20, std::min<int64_t>(4 * n, (int64_t)1 << 25))) {
+    // one stable counting pass over (left, right, conj flag): the steps are
+    // in id order, so equal keys keep it
+    std::vector<uint32_t> key(steps.size());
+    std::vector<uint32_t> start((size_t)n_keys + 1, 0);
+    for (size_t k = 0; k < steps.size(); ++k) {
+      const PlanStep& s = steps[k];
+      key[k] = (uint32_t)(((int64_t)s.left.product * P + s.right.product) * 2 + rc(s));
+      ++start[key[k] + 1];
+    }
+    for (int64_t i = 0; i < n_keys; ++i) start[(size_t)i + 1] += start[(size_t)i];
+    std::vector<PlanStep> sorted(steps.size());
+    for (size_t k = 0; k < steps.size(); ++k) sorted[start[key[k]]++] = steps[k];
+    return sorted;
+  }
+  if (n_keys < ((int64_t)1 << 32)) {
     std::vector<uint64_t> key;
     key.reserve(steps.size());
     for (const PlanStep& s : steps)

@@ -147,6 +147,37 @@ def test_zincblende_integrals(prec, sched):
     assert np.max(err) < TOL[prec], np.max(err)
 
 
+@pytest.mark.parametrize("prec,sched", [COMBOS[0], COMBOS[1], COMBOS[4]], ids=[IDS[0], IDS[1], IDS[4]])
+def test_stale_values(prec, sched):
+    """CLAT_FLAG_STALE_VALUES (coulombo.cpp:194-205): the value is kept across
+    a change of left product when the right product and the conj flag repeat.
+    This plan has one such step: (1,4,3,4) follows (3,3,2,4) with right
+    product 8 and conj flag 1 under a new left product, so it takes that
+    value (conjugated, its left product being conjugate)."""
+    print("case stale", prec, sched, flush=True)
+    w = zb.workload(2, 2, 2, 2, 2, config=0, sigma_frac=0.2, n_orb=4)
+    X, Y, Z = w["dims"]
+    quads = [(2, 2, 1, 2), (3, 2, 4, 3), (1, 4, 4, 4), (1, 4, 3, 4), (3, 3, 2, 4), (4, 3, 2, 3),
+             (3, 1, 1, 2), (3, 1, 2, 1)]
+    eng = clat.Engine((X, Y, Z), prec, sched)
+    eng.initialize(clat.InteractionSimple(w["step"], 0.0, 10.89))
+    eng.set_atoms(w["cells"], 4)
+    for s in w["states"]:
+        eng.add_state(s)
+    got_stale = eng.compute_integrals(quads, stale_values=True)
+    got = eng.compute_integrals(quads)
+    rho = O.products(np.stack(w["states"]))
+    Gh = O.ghat(O.gtable(X, Y, Z, w["step"], 0.0, 10.89))
+    want_stale = O.compute_integrals(quads, 4, rho, w["cells"], (X, Y, Z), Gh, stale=True)
+    want = O.compute_integrals(quads, 4, rho, w["cells"], (X, Y, Z), Gh)
+    changed = np.abs(want_stale - want) > 1e-6 * np.max(np.abs(want))
+    assert list(np.nonzero(changed)[0]) == [3]
+    assert want_stale[3] == np.conj(want_stale[4])
+    assert _rel(got_stale, want_stale) < TOL[prec], _rel(got_stale, want_stale)
+    assert _rel(got, want) < TOL[prec], _rel(got, want)
+    assert got_stale[3] == np.conj(got_stale[4])
+
+
 @pytest.mark.parametrize("prec", [clat.PREC_FP64, clat.PREC_FP32, clat.PREC_FP32X], ids=["fp64", "fp32", "fp32x"])
 def test_potential_at_atoms(prec):
     """potentials.cpp:140-158: V of each self product at every atom's cell."""

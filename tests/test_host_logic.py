@@ -108,8 +108,10 @@ def test_plan_vs_restatement(spec, n, nh):
 @pytest.mark.parametrize("n", [40, 320])
 def test_plan_large_state_counts(n):
     """Many states: the product graph of 40 states goes through the edge
-    bitmap and the radix-sorted steps; 320 states (P^2 past both limits)
-    through the sorted edge list and the comparison sort."""
+    bitmap and the radix-sorted steps (2 P^2 keys, more than 2^20 and than
+    four per step); 320 states (P^2 past both limits) through the sorted
+    edge list and the comparison sort.  Smaller key ranges, as in the
+    restatement tests above, take one counting pass."""
     rng = np.random.default_rng(n)
     quads = rng.integers(1, n + 1, size=(3000, 4)).astype(np.int32)
     got = clat.plan(quads, n)
