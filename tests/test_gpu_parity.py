@@ -178,6 +178,80 @@ def test_stale_values(prec, sched):
     assert got_stale[3] == np.conj(got_stale[4])
 
 
+@pytest.mark.parametrize("prec,sched", [COMBOS[0], COMBOS[1], COMBOS[4]], ids=[IDS[0], IDS[1], IDS[4]])
+def test_atoms_sharing_cells(prec, sched):
+    """Product.cpp:16-30 assigns each atom's density to its cell, so when
+    atoms share a cell the last one in file order is the one that counts.
+    Twelve extra atoms, interleaved at random positions, land in cells that
+    are already occupied; integrals and per-atom potentials vs the oracle
+    built from the last atom of every cell."""
+    print("case shared cells", prec, sched, flush=True)
+    w = zb.workload(2, 2, 2, 2, 2, config=0, sigma_frac=0.2, n_orb=4)
+    X, Y, Z = w["dims"]
+    rng = np.random.default_rng(7)
+    n0 = len(w["cells"])
+    extra = rng.choice(n0, 12, replace=False)
+    # atom order: the originals with the extras inserted at random places
+    order = list(range(n0))
+    for k, src in enumerate(extra):
+        order.insert(int(rng.integers(0, len(order) + 1)), n0 + k)
+    cells_ext = np.concatenate([w["cells"], w["cells"][extra]])
+    states_ext = [np.concatenate([s, rng.standard_normal((12, 4)) + 1j * rng.standard_normal((12, 4))])
+                  for s in w["states"]]
+    cells = cells_ext[order].astype(np.int32)
+    states = [s[order] for s in states_ext]
+    last = {}
+    for a, c in enumerate(cells):
+        last[int(c)] = a
+    eff = np.array(sorted(last.values()))
+    assert len(eff) == n0 and len(cells) == n0 + 12
+    quads = clat.pattern_enumerate("****", 4, 2)
+    eng = clat.Engine((X, Y, Z), prec, sched)
+    eng.initialize(clat.InteractionSimple(w["step"], 0.0, 10.89))
+    eng.set_atoms(cells, 4)
+    for s in states:
+        eng.add_state(s)
+    got = eng.compute_integrals(quads)
+    st = np.stack(states)
+    rho = O.products(st[:, eff, :])
+    Gh = O.ghat(O.gtable(X, Y, Z, w["step"], 0.0, 10.89))
+    want = O.compute_integrals([tuple(q) for q in quads], 4, rho, cells[eff], (X, Y, Z), Gh)
+    assert _rel(got, want) < TOL[prec], _rel(got, want)
+    # the same run with every atom counted differs: the collisions matter here
+    naive = O.compute_integrals([tuple(q) for q in quads], 4, O.products(st), cells, (X, Y, Z), Gh)
+    assert _rel(naive, want) > 1e-3
+    # potential of state 1's density at every atom, overwritten ones included
+    v = eng.potential_at_atoms(1)
+    F = np.zeros(X * Y * Z, complex)
+    F[cells[eff]] = np.sum(np.abs(st[0, eff, :]) ** 2, axis=1)
+    Vo = O.prepare(F.reshape(Z, Y, X), Gh).reshape(-1)[cells]
+    assert np.max(np.abs(v - Vo)) / np.max(np.abs(Vo)) < TOL[prec]
+
+
+def test_empty_and_invalid_integral_lists():
+    """No integrals (an empty list, or a pattern that selects none: the
+    reference then writes no files) gives an empty result; a state index past
+    the loaded states is an error, and the engine stays usable after it."""
+    print("case empty lists", flush=True)
+    w = zb.workload(2, 2, 2, 1, 1, config=0, sigma_frac=0.2, n_orb=4)
+    X, Y, Z = w["dims"]
+    eng = clat.Engine((X, Y, Z), clat.PREC_FP32X, clat.SCHED_SHARED)
+    eng.initialize(clat.InteractionSimple(w["step"], 0.0, 10.89))
+    eng.set_atoms(w["cells"], 4)
+    for s in w["states"]:
+        eng.add_state(s)
+    assert eng.compute_integrals(np.zeros((0, 4), np.int32)).shape == (0,)
+    none = clat.pattern_enumerate("9999", 2, 1)
+    assert len(none) == 0 and eng.compute_integrals(none).shape == (0,)
+    with pytest.raises(clat.ClatError):
+        eng.compute_integrals([(1, 2, 3, 1)])
+    got = eng.compute_integrals([(1, 2, 2, 1), (2, 1, 1, 2)])
+    rho = O.products(np.stack(w["states"]))
+    Gh = O.ghat(O.gtable(X, Y, Z, w["step"], 0.0, 10.89))
+    want = O.compute_integrals([(1, 2, 2, 1), (2, 1, 1, 2)], 2, rho, w["cells"], (X, Y, Z), Gh)
+    assert _rel(got, want) < TOL[clat.PREC_FP32X]
+
+
 @pytest.mark.parametrize("prec", [clat.PREC_FP64, clat.PREC_FP32, clat.PREC_FP32X], ids=["fp64", "fp32", "fp32x"])
 def test_potential_at_atoms(prec):
     """potentials.cpp:140-158: V of each self product at every atom's cell."""
