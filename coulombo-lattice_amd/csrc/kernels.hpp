@@ -319,15 +319,30 @@ __global__ __launch_bounds__(NT) void k_dct1(double* __restrict__ G, AxisPlan p,
   const int nC = min(2 * C, ncols - c0);
   double* base = G + (size_t)o * os + (size_t)c0 * cs;
   const int tot = C * L2;
-  for (int e = threadIdx.x; e < tot; e += NT) {
-    int c, m;
-    if (LM) { c = (int)fdiv((uint32_t)e, p.l_magic); m = e - c * L2; }
-    else { c = e & (C - 1); m = e >> lgC; }
-    const int i = m <= M ? m : L2 - m;
-    double v0 = 0.0, v1 = 0.0;
-    if (c < nC) v0 = base[(size_t)i * is + (size_t)c * cs];
-    if (c + C < nC) v1 = base[(size_t)i * is + (size_t)(c + C) * cs];
-    buf[LM ? c * L2 + m : m * C + c] = make_double2(v0, v1);
+  // loads in batches of UB per thread, all issued before the first LDS write
+  constexpr int UB = 8;
+  for (int e0 = threadIdx.x; e0 < tot; e0 += NT * UB) {
+    double v0[UB], v1[UB];
+    int dst[UB];
+#pragma unroll
+    for (int u = 0; u < UB; ++u) {
+      const int e = e0 + u * NT;
+      v0[u] = 0.0;
+      v1[u] = 0.0;
+      dst[u] = -1;
+      if (e < tot) {
+        int c, m;
+        if (LM) { c = (int)fdiv((uint32_t)e, p.l_magic); m = e - c * L2; }
+        else { c = e & (C - 1); m = e >> lgC; }
+        const int i = m <= M ? m : L2 - m;
+        if (c < nC) v0[u] = base[(size_t)i * is + (size_t)c * cs];
+        if (c + C < nC) v1[u] = base[(size_t)i * is + (size_t)(c + C) * cs];
+        dst[u] = LM ? c * L2 + m : m * C + c;
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < UB; ++u)
+      if (dst[u] >= 0) buf[dst[u]] = make_double2(v0[u], v1[u]);
   }
   __syncthreads();
   lds_fft<false, EMAX, LM, double>(buf, tw, p, C, lgC);
