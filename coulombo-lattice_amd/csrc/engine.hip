@@ -269,7 +269,8 @@ struct clat_engine {
   const double2* b_in2[LINE_NB] = {};  // second real density packed into the prepare (or null)
   double2* b_out2[LINE_NB] = {};
   bool pack = true;                     // pairs of real (diagonal) densities share a prepare
-  int ovl_mode = 0;                     // overlap kernel: 0 by size, 1 vector FMA, 2 FP64 MFMA
+  int n_cu = 256;     // compute units of the device
+  int ovl_mode = 0;  // overlap kernel: 0 by size, 1 vector FMA, 2 FP64 MFMA, 3 vector FMA in 48 x 48 tiles
   bool table_pending = false;           // stage_ms[0] of the last table build not read yet
   size_t a_set = 0, b_set = 0, d_set = 0;  // elements of A, B, D per prepare
   bool xline = false;                // x passes on the line kernel, y passes skip empty lines
@@ -1205,6 +1206,8 @@ int clat_create(int X, int Y, int Z, int precision, int schedule, clat_engine** 
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0)
       throw host::Error(CLAT_EDEVICE, "no HIP device available");
     HIPX(hipGetDevice(&e->device));
+    HIPX(hipDeviceGetAttribute(&e->n_cu, hipDeviceAttributeMultiprocessorCount, e->device));
+    e->n_cu = std::max(1, e->n_cu);
     e->X = X; e->Y = Y; e->Z = Z;
     e->Xi = engine_len(X); e->Yi = engine_len(Y); e->Zi = engine_len(Z);
     if ((double)e->Xi * e->Yi * e->Zi * 4.0 >= 4294967295.0)
@@ -1350,7 +1353,8 @@ int clat_create(int X, int Y, int Z, int precision, int schedule, clat_engine** 
     if (const char* g = std::getenv("CLAT_BATCH")) e->nb_max = std::max(1, std::min(e->nb_max, std::atoi(g)));
     if (const char* g = std::getenv("CLAT_PACK")) e->pack = std::atoi(g) != 0;
     if (const char* g = std::getenv("CLAT_OVL"))
-      e->ovl_mode = std::strcmp(g, "valu") == 0 ? 1 : std::strcmp(g, "mfma") == 0 ? 2 : 0;
+      e->ovl_mode = std::strcmp(g, "valu") == 0 ? 1 : std::strcmp(g, "mfma") == 0 ? 2
+                    : std::strcmp(g, "valu48") == 0 ? 3 : 0;
     e->a_bytes = e->nb_max * e->a_set * e->s_a;
     e->A = e->dalloc(e->a_bytes);
     if (e->shared()) {
@@ -1699,24 +1703,30 @@ int clat_compute_integrals(clat_engine* e, const int32_t* quads, int64_t n, int 
     }
     HIPX(hipEventRecord(e->stage_ev[3], e->stream));
     // overlaps of every (slot, product) pair
-    // tile edge 3 TX with TX x TX threads (TX = 16 measured fastest for tens of products);
-    // 64 x 64 tiles on FP64 MFMA when both sides fill them
-    constexpr int TX = 16;
+    // 64 x 64 tiles on FP64 MFMA when both sides fill them; else 36 x 36
+    // tiles on vector FMA when both sides fit one, 48 x 48 otherwise
     const bool mfma = e->ovl_mode == 2 || (e->ovl_mode == 0 && nL >= OVM_MIN && P >= OVM_MIN);
-    const int OV_T = mfma ? OVM_T : 3 * TX;
-    const int ovs = mfma ? OVM_S : OV_S;
+    const bool t36 = !mfma && e->ovl_mode != 3 && nL <= 36 && P <= 36;
+    const int OV_T = mfma ? OVM_T : t36 ? 36 : 48;
+    const int ovs = mfma ? OVM_S : t36 ? 24 : OV_S;
+    const int groups = t36 ? 3 : 1;  // site groups per block, a partial slice each
     const int tl = (nL + OV_T - 1) / OV_T, tr = (P + OV_T - 1) / OV_T;
-    int64_t n_chunks = std::max<int64_t>(1, 2048 / std::max(1, tl * tr));
+    // blocks: about 2048 for the 48 x 48 and MFMA tiles; for the 36 x 36
+    // tiles (246 VGPRs: two blocks per CU) one full round, two per CU
+    // (0.33 ms against 0.37 at 2.7 blocks per CU and 0.45 at 8, 1M atoms)
+    const int want_blocks = t36 ? 2 * e->n_cu : 2048;
+    int64_t n_chunks = std::max<int64_t>(1, want_blocks / std::max(1, tl * tr));
     n_chunks = std::min<int64_t>(n_chunks, (e->n_sites + ovs - 1) / ovs);
     n_chunks = std::max<int64_t>(1, n_chunks);
     int64_t chunk = (e->n_sites + n_chunks - 1) / n_chunks;
     chunk = std::max<int64_t>(ovs, (chunk + ovs - 1) / ovs * ovs);
     n_chunks = std::max<int64_t>(1, (e->n_sites + chunk - 1) / chunk);
+    const int64_t n_slices = n_chunks * groups;
     const size_t n_out = (size_t)std::max(1, nL) * P * 4;
-    if (e->part_cap < n_out * n_chunks) {
+    if (e->part_cap < n_out * n_slices) {
       e->dfree(e->part, e->part_cap * sizeof(double));
-      e->part = (double*)e->dalloc(n_out * n_chunks * sizeof(double));
-      e->part_cap = n_out * n_chunks;
+      e->part = (double*)e->dalloc(n_out * n_slices * sizeof(double));
+      e->part_cap = n_out * n_slices;
     }
     if (e->ovl_cap < n_out) {
       e->dfree(e->ovl, e->ovl_cap * sizeof(double));
@@ -1730,12 +1740,15 @@ int clat_compute_integrals(clat_engine* e, const int32_t* quads, int64_t n, int 
       if (mfma)
         hipLaunchKernelGGL(k_overlap_partial_mfma, og, dim3(256), 0, e->stream, e->Vat, e->rho, nL, P, e->n_sites,
                            chunk, e->part);
+      else if (t36)
+        hipLaunchKernelGGL((k_overlap_partial<9, 4, 3, 24>), og, dim3(243), 0, e->stream, e->Vat, e->rho, nL, P,
+                           e->n_sites, chunk, e->part);
       else
-        hipLaunchKernelGGL(k_overlap_partial<TX>, og, dim3(TX * TX), 0, e->stream, e->Vat, e->rho, nL, P,
+        hipLaunchKernelGGL((k_overlap_partial<16, 3, 1, OV_S>), og, dim3(256), 0, e->stream, e->Vat, e->rho, nL, P,
                            e->n_sites, chunk, e->part);
       LAUNCH_CHECK();
       hipLaunchKernelGGL(k_overlap_reduce, dim3((unsigned)((n_out + 63) / 64)), dim3(1024), 0, e->stream, e->part,
-                         n_chunks, (int64_t)n_out, e->ovl);
+                         n_slices, (int64_t)n_out, e->ovl);
       LAUNCH_CHECK();
       e->kend(K_OVERLAP, (double)(tl * tr) * (double)(2 * OV_T) * (double)e->n_sites * 16.0);
       HIPX(hipMemcpyAsync(ov.data(), e->ovl, n_out * sizeof(double), hipMemcpyDeviceToHost, e->stream));

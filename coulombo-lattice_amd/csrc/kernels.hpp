@@ -570,32 +570,37 @@ __global__ void k_products_direct(const double2* __restrict__ coeffs, int64_t n_
 // the sites are formed,
 //   A = sum Vr*pr, B = sum Vi*pi, C = sum Vr*pi, D = sum Vi*pr,
 // from which  sum V*rho = (A-B) + i(C+D)  and  sum V*conj(rho) = (A+B) + i(D-C).
-// Stage 1: a 48x48 (slot, product) tile per block, 3x3 pairs per thread, over
-// one chunk of sites; partial sums to part[chunk][l][r][4].
+// Stage 1: a T x T (slot, product) tile per block, PT x PT pairs per thread,
+// over one chunk of sites; partial sums to part[slice][l][r][4].  Two shapes:
+// TX = 16, PT = 3 (48 x 48, one group of 256 threads), and TX = 9, PT = 4
+// (36 x 36 without idle pairs for up to 36 slots and products, as eight
+// states give; three groups of 81 threads, each summing every third staged
+// site into its own slice, so slice = chunk * G + group).
 // ---------------------------------------------------------------------------
-constexpr int OV_S = 16;   // sites staged per step
+constexpr int OV_S = 16;   // sites staged per step (48 x 48 tiles)
 
-// TX x TX threads, 3 x 3 (left, right) pairs each: a tile edge of 3 TX.
-template <int TX>
-__global__ __launch_bounds__(TX * TX) void k_overlap_partial(const double2* __restrict__ Vat,
-                                                             const double2* __restrict__ rho,
-                                                             int n_left, int n_right, int64_t n_sites,
-                                                             int64_t chunk, double* __restrict__ part) {
-  constexpr int T = 3 * TX, NTH = TX * TX;
-  __shared__ double2 sv[T][OV_S + 1];
-  __shared__ double2 sr[T][OV_S + 1];
-  const int tx = threadIdx.x % TX, ty = threadIdx.x / TX;
+template <int TX, int PT, int G, int OVS>
+__global__ __launch_bounds__(TX * TX * G) void k_overlap_partial(const double2* __restrict__ Vat,
+                                                                 const double2* __restrict__ rho,
+                                                                 int n_left, int n_right, int64_t n_sites,
+                                                                 int64_t chunk, double* __restrict__ part) {
+  static_assert(OVS % G == 0, "each group takes the same number of staged sites");
+  constexpr int T = PT * TX, NG = TX * TX, NTH = NG * G;
+  __shared__ double2 sv[T][OVS + 1];
+  __shared__ double2 sr[T][OVS + 1];
+  const int g = threadIdx.x / NG, t = threadIdx.x - g * NG;
+  const int tx = t % TX, ty = t / TX;
   const int l0 = blockIdx.x * T, r0 = blockIdx.y * T;
   const int64_t s0 = (int64_t)blockIdx.z * chunk;
   const int64_t s1 = min(n_sites, s0 + chunk);
-  double acc[3][3][4];
+  double acc[PT][PT][4];
 #pragma unroll
-  for (int i = 0; i < 3; ++i)
+  for (int i = 0; i < PT; ++i)
 #pragma unroll
-    for (int j = 0; j < 3; ++j)
+    for (int j = 0; j < PT; ++j)
 #pragma unroll
       for (int q = 0; q < 4; ++q) acc[i][j][q] = 0.0;
-  constexpr int NE = (T * OV_S + NTH - 1) / NTH;  // staged elements per thread and table
+  constexpr int NE = (T * OVS + NTH - 1) / NTH;  // staged elements per thread and table
   // software pipeline: the next batch of sites is loaded into registers while
   // the current one is summed from LDS
   double2 bv[NE], br[NE];
@@ -603,35 +608,35 @@ __global__ __launch_bounds__(TX * TX) void k_overlap_partial(const double2* __re
 #pragma unroll
     for (int j = 0; j < NE; ++j) {
       const int e = threadIdx.x + j * NTH;
-      const int row = e / OV_S, col = e - row * OV_S;
+      const int row = e / OVS, col = e - row * OVS;
       const int64_t site = sb + col;
-      const bool ok = e < T * OV_S && site < s1;
+      const bool ok = e < T * OVS && site < s1;
       bv[j] = (ok && l0 + row < n_left) ? nt_load(Vat + (size_t)(l0 + row) * n_sites + site) : make_double2(0.0, 0.0);
       br[j] = (ok && r0 + row < n_right) ? nt_load(rho + (size_t)(r0 + row) * n_sites + site) : make_double2(0.0, 0.0);
     }
   };
   if (s0 < s1) fetch(s0);
-  for (int64_t sb = s0; sb < s1; sb += OV_S) {
+  for (int64_t sb = s0; sb < s1; sb += OVS) {
 #pragma unroll
     for (int j = 0; j < NE; ++j) {
       const int e = threadIdx.x + j * NTH;
-      if (e < T * OV_S) {
-        const int row = e / OV_S, col = e - row * OV_S;
+      if (e < T * OVS) {
+        const int row = e / OVS, col = e - row * OVS;
         sv[row][col] = bv[j];
         sr[row][col] = br[j];
       }
     }
     __syncthreads();
-    if (sb + OV_S < s1) fetch(sb + OV_S);
+    if (sb + OVS < s1) fetch(sb + OVS);
 #pragma unroll 2
-    for (int s = 0; s < OV_S; ++s) {
-      double2 v[3], r[3];
+    for (int s = g; s < OVS; s += G) {
+      double2 v[PT], r[PT];
 #pragma unroll
-      for (int i = 0; i < 3; ++i) { v[i] = sv[ty + TX * i][s]; r[i] = sr[tx + TX * i][s]; }
+      for (int i = 0; i < PT; ++i) { v[i] = sv[ty + TX * i][s]; r[i] = sr[tx + TX * i][s]; }
 #pragma unroll
-      for (int i = 0; i < 3; ++i)
+      for (int i = 0; i < PT; ++i)
 #pragma unroll
-        for (int j = 0; j < 3; ++j) {
+        for (int j = 0; j < PT; ++j) {
           acc[i][j][0] = fma(v[i].x, r[j].x, acc[i][j][0]);
           acc[i][j][1] = fma(v[i].y, r[j].y, acc[i][j][1]);
           acc[i][j][2] = fma(v[i].x, r[j].y, acc[i][j][2]);
@@ -640,13 +645,14 @@ __global__ __launch_bounds__(TX * TX) void k_overlap_partial(const double2* __re
     }
     __syncthreads();
   }
+  const size_t slice = (size_t)blockIdx.z * G + g;
 #pragma unroll
-  for (int i = 0; i < 3; ++i)
+  for (int i = 0; i < PT; ++i)
 #pragma unroll
-    for (int j = 0; j < 3; ++j) {
+    for (int j = 0; j < PT; ++j) {
       const int l = l0 + ty + TX * i, r = r0 + tx + TX * j;
       if (l < n_left && r < n_right) {
-        double* d = part + (((size_t)blockIdx.z * n_left + l) * n_right + r) * 4;
+        double* d = part + ((slice * n_left + l) * n_right + r) * 4;
 #pragma unroll
         for (int q = 0; q < 4; ++q) d[q] = acc[i][j][q];
       }

@@ -355,6 +355,11 @@ def test_batched_prepares(prec, monkeypatch):
     assert _rel(on_mfma, got) < 1e-12, _rel(on_mfma, got)
     err = np.abs(on_mfma - want) / np.maximum(np.abs(want), 1e-3 * scale)
     assert np.max(err) < TOL[prec], np.max(err)
+    # the default here is the 36 x 36 vector tile (three site groups per
+    # block); the 48 x 48 tile sums the same terms in another order
+    monkeypatch.setenv("CLAT_OVL", "valu48")
+    on48 = run()
+    assert _rel(on48, got) < 1e-12, _rel(on48, got)
 
 
 @pytest.mark.parametrize("prec", [clat.PREC_FP64, clat.PREC_FP32X], ids=["fp64", "fp32x"])
