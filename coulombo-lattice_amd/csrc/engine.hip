@@ -9,6 +9,7 @@
 #include <cstring>
 #include <array>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/clat.h"
@@ -1641,6 +1642,12 @@ int clat_compute_integrals(clat_engine* e, const int32_t* quads, int64_t n, int 
   if (!e->initialized) return set_err(CLAT_ESTATE, "calculator is not initialized");
   if (e->n_states <= 0) return set_err(CLAT_ESTATE, "no states loaded");
   return guarded([&] {
+    // the pair densities do not depend on the plan: queued first, they run
+    // while the host plans
+    HIPX(hipEventRecord(e->stage_ev[1], e->stream));
+    e->rho_states = -1;  // the pair densities are part of every run's compute stage
+    build_products(e);
+    HIPX(hipEventRecord(e->stage_ev[2], e->stream));
     // the stale-values bookkeeping walks the steps in execution order; without
     // it every value depends on its own step only and id order serves
     const bool stale = (flags & CLAT_FLAG_STALE_VALUES) != 0;
@@ -1655,10 +1662,6 @@ int clat_compute_integrals(clat_engine* e, const int32_t* quads, int64_t n, int 
         slots.push_back(p);
       }
     const int nL = (int)slots.size();
-    HIPX(hipEventRecord(e->stage_ev[1], e->stream));
-    e->rho_states = -1;  // the pair densities are part of every run's compute stage
-    build_products(e);
-    HIPX(hipEventRecord(e->stage_ev[2], e->stream));
     const size_t vat_need = (size_t)std::max(1, nL) * (size_t)std::max<int64_t>(1, e->n_sites);
     if (e->vat_cap < vat_need) {
       e->dfree(e->Vat, e->vat_cap * sizeof(double2));
@@ -1757,20 +1760,47 @@ int clat_compute_integrals(clat_engine* e, const int32_t* quads, int64_t n, int 
     HIPX(hipStreamSynchronize(e->stream));
     e->kcollect();
     // main loop bookkeeping (coulombo.cpp:186-209)
-    int lastR = -1, lastRc = 0;
-    double vre = 0.0, vim = 0.0;
-    for (const auto& s : steps) {
-      const int rc = (s.left.conj != s.right.conj) ? 1 : 0;
-      if (!stale || s.right.product != lastR || rc != lastRc) {
-        const double* q = ov.data() + (((size_t)slot_of[(size_t)s.left.product]) * P + s.right.product) * 4;
-        const double Av = q[0], Bv = q[1], Cv = q[2], Dv = q[3];
-        if (rc) { vre = Av + Bv; vim = Dv - Cv; }
-        else { vre = Av - Bv; vim = Cv + Dv; }
-        lastR = s.right.product;
-        lastRc = rc;
+    auto value_of = [&](const host::PlanStep& s, int rc, double& vre, double& vim) {
+      const double* q = ov.data() + (((size_t)slot_of[(size_t)s.left.product]) * P + s.right.product) * 4;
+      const double Av = q[0], Bv = q[1], Cv = q[2], Dv = q[3];
+      if (rc) { vre = Av + Bv; vim = Dv - Cv; }
+      else { vre = Av - Bv; vim = Cv + Dv; }
+    };
+    if (!stale) {
+      // every value from its own step: up to 16 host threads over id ranges
+      auto run = [&](size_t k0, size_t k1) {
+        for (size_t k = k0; k < k1; ++k) {
+          const auto& s = steps[k];
+          double vre, vim;
+          value_of(s, (s.left.conj != s.right.conj) ? 1 : 0, vre, vim);
+          out[2 * (size_t)s.id] = vre;
+          out[2 * (size_t)s.id + 1] = s.left.conj ? -vim : vim;
+        }
+      };
+      const size_t ns = steps.size();
+      const size_t nt = std::min<size_t>({(size_t)16, (size_t)std::max(1u, std::thread::hardware_concurrency()),
+                                          std::max<size_t>(1, ns / 65536)});
+      if (nt <= 1) {
+        run(0, ns);
+      } else {
+        std::vector<std::thread> pool;
+        for (size_t t = 1; t < nt; ++t) pool.emplace_back(run, ns * t / nt, ns * (t + 1) / nt);
+        run(0, ns / nt);
+        for (auto& th : pool) th.join();
       }
-      out[2 * (size_t)s.id] = vre;
-      out[2 * (size_t)s.id + 1] = s.left.conj ? -vim : vim;
+    } else {
+      int lastR = -1, lastRc = 0;
+      double vre = 0.0, vim = 0.0;
+      for (const auto& s : steps) {
+        const int rc = (s.left.conj != s.right.conj) ? 1 : 0;
+        if (s.right.product != lastR || rc != lastRc) {
+          value_of(s, rc, vre, vim);
+          lastR = s.right.product;
+          lastRc = rc;
+        }
+        out[2 * (size_t)s.id] = vre;
+        out[2 * (size_t)s.id + 1] = s.left.conj ? -vim : vim;
+      }
     }
     float ms;
     HIPX(hipEventElapsedTime(&ms, e->stage_ev[1], e->stage_ev[2])); e->stage_ms[1] = ms;

@@ -13,6 +13,7 @@
 #include <set>
 #include <stdexcept>
 #include <string>
+#include <thread>
 #include <utility>
 #include <vector>
 
@@ -291,24 +292,65 @@ inline PlanProduct plan_product(int iL, int iR) {
 
 // ordered = false: the steps in id order after the cover swap, for callers
 // that need each step's products but not the execution order
+// fn(k0, k1, t) over [0, n) in contiguous ranges, one per thread t: up to
+// 16 host threads, one per 65536 items (the calling thread takes range 0)
+template <typename F>
+inline size_t parallel_ranges(size_t n, F&& fn) {
+  const size_t nt = std::min<size_t>({(size_t)16, (size_t)std::max(1u, std::thread::hardware_concurrency()),
+                                      std::max<size_t>(1, n / 65536)});
+  if (nt <= 1) {
+    fn((size_t)0, n, (size_t)0);
+    return 1;
+  }
+  std::vector<std::thread> pool;
+  for (size_t t = 1; t < nt; ++t) pool.emplace_back([&fn, n, nt, t] { fn(n * t / nt, n * (t + 1) / nt, t); });
+  fn((size_t)0, n / nt, (size_t)0);
+  for (auto& th : pool) th.join();
+  return nt;
+}
+
 inline std::vector<PlanStep> make_plan(const int32_t* quads, int64_t n, int n_states, bool ordered = true) {
   if (n > std::numeric_limits<int>::max()) throw Error(1, "too many integrals to compute");
   std::vector<PlanStep> steps((size_t)n);
   const int64_t P = (int64_t)n_states * (n_states + 1) / 2;
-  for (int64_t k = 0; k < n; ++k) {
-    const int32_t* q = quads + 4 * k;
-    for (int j = 0; j < 4; ++j)
-      if (q[j] < 1 || q[j] > n_states) throw Error(1, "integral state index out of range");
-    steps[(size_t)k] = PlanStep{(int)k, plan_product(q[0], q[3]), plan_product(q[1], q[2])};
-  }
+  std::vector<char> bad(16, 0);  // a range check failed in thread t
+  parallel_ranges((size_t)n, [&](size_t k0, size_t k1, size_t t) {
+    for (size_t k = k0; k < k1; ++k) {
+      const int32_t* q = quads + 4 * k;
+      for (int j = 0; j < 4; ++j)
+        if (q[j] < 1 || q[j] > n_states) {
+          bad[t] = 1;
+          return;
+        }
+      steps[k] = PlanStep{(int)k, plan_product(q[0], q[3]), plan_product(q[1], q[2])};
+    }
+  });
+  for (char b : bad)
+    if (b) throw Error(1, "integral state index out of range");
   // the product graph's edges, sorted and unique: a bitmap over P x P when
   // it is small, else a sort of the edge keys
   std::vector<std::pair<int, int>> edges;
   if (P * P <= ((int64_t)1 << 28)) {
-    std::vector<uint64_t> bits((size_t)((P * P + 63) / 64), 0);
-    for (const PlanStep& s : steps) {
-      const uint64_t e = (uint64_t)s.left.product * P + s.right.product;
-      bits[e >> 6] |= (uint64_t)1 << (e & 63);
+    const size_t words = (size_t)((P * P + 63) / 64);
+    std::vector<uint64_t> bits(words, 0);
+    // per-thread bitmaps, OR-merged, while they stay small
+    const size_t nt_max = words <= ((size_t)1 << 16) ? 16 : 1;
+    std::vector<std::vector<uint64_t>> part(nt_max > 1 ? nt_max : 0);
+    auto mark = [&](size_t k0, size_t k1, uint64_t* b) {
+      for (size_t k = k0; k < k1; ++k) {
+        const uint64_t e = (uint64_t)steps[k].left.product * P + steps[k].right.product;
+        b[e >> 6] |= (uint64_t)1 << (e & 63);
+      }
+    };
+    if (nt_max > 1) {
+      const size_t nt = parallel_ranges(steps.size(), [&](size_t k0, size_t k1, size_t t) {
+        part[t].assign(words, 0);
+        mark(k0, k1, part[t].data());
+      });
+      for (size_t t = 0; t < nt; ++t)
+        for (size_t w = 0; w < words; ++w) bits[w] |= part[t][w];
+    } else {
+      mark(0, steps.size(), bits.data());
     }
     for (size_t w = 0; w < bits.size(); ++w)
       for (uint64_t b = bits[w]; b; b &= b - 1) {
@@ -326,8 +368,10 @@ inline std::vector<PlanStep> make_plan(const int32_t* quads, int64_t n, int n_st
   const std::vector<int> cv = vertex_cover((int)P, edges);
   std::vector<char> in_cover((size_t)P, 0);
   for (int v : cv) in_cover[(size_t)v] = 1;
-  for (PlanStep& s : steps)
-    if (!in_cover[(size_t)s.left.product]) std::swap(s.left, s.right);
+  parallel_ranges(steps.size(), [&](size_t k0, size_t k1, size_t) {
+    for (size_t k = k0; k < k1; ++k)
+      if (!in_cover[(size_t)steps[k].left.product]) std::swap(steps[k].left, steps[k].right);
+  });
   if (!ordered) return steps;
   // order (Planner.hpp:94-101): left product, right product, then the steps
   // whose two products have equal conj flags before the others

@@ -121,6 +121,23 @@ def test_plan_large_state_counts(n):
     assert [key(*r[1:]) for r in got] == [key(L, lc, R, rc) for _, (L, lc), (R, rc) in want]
 
 
+def test_plan_threaded_ranges():
+    """200000 integrals: the steps, the edge bitmap and the cover swaps are
+    built by several host threads over id ranges; same plan as the
+    restatement, and a bad state index inside a later range is still an
+    error."""
+    rng = np.random.default_rng(12)
+    quads = rng.integers(1, 13, size=(200000, 4)).astype(np.int32)
+    got = clat.plan(quads, 12)
+    want = O.plan([tuple(int(v) for v in q) for q in quads], 12)
+    assert _plan_key(got) == _plan_key([(k, L, lc, R, rc) for k, (L, lc), (R, rc) in want])
+    key = lambda L, lc, R, rc: (L, R, 0 if lc == rc else 1)  # noqa: E731
+    assert [key(*r[1:]) for r in got] == [key(L, lc, R, rc) for _, (L, lc), (R, rc) in want]
+    quads[170000, 1] = 13
+    with pytest.raises(clat.ClatError):
+        clat.plan(quads, 12)
+
+
 def test_plan_product_indexing():
     """Planner.hpp:134-144: product(iL, iR) = iL(iL-1)/2 + iR - 1 with swap+conj."""
     rows = clat.plan([(1, 2, 3, 4)], 4)
