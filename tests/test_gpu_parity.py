@@ -306,7 +306,7 @@ def test_site_lines_match_oracle(prec, dims):
 
 @pytest.mark.parametrize("prec", [clat.PREC_FP64, clat.PREC_FP32, clat.PREC_FP32X], ids=["fp64", "fp32", "fp32x"])
 def test_batched_prepares(prec, monkeypatch):
-    """Several left products per launch (up to four): 5 states on a 32^3 grid
+    """Several left products per launch (up to eight): 5 states on a 32^3 grid
     give a cover of 15 left products, not a multiple of the batch; the values
     match the oracle main loop and are bit-identical to one prepare per launch."""
     print("case batched", prec, flush=True)
