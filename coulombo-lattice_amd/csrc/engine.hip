@@ -1064,16 +1064,22 @@ void build_products(clat_engine* e) {
   // S sites per block (power of two, <= 64): all states of S + 1 sites in at
   // most 22.5 KB of LDS, so that six blocks share a CU (8 states: 0.59 ms per
   // step against 0.72 with 44 KB tiles), but at least two sites, or too few
-  // threads load (32 states: 2.96 ms against 4.10 with one site)
+  // threads load (32 states: 2.96 ms against 4.10 with one site), and four
+  // where they fit 64 KB (32 states x 20 orbitals: 1.6 against 2.8 ms with two)
   const size_t per_site = (size_t)e->n_states * e->n_orb * sizeof(double2);
   int S = 64;
   while (S > 1 && (size_t)(S + 1) * per_site > 23040) S /= 2;
   if (S == 1 && 3 * per_site <= 45056) S = 2;
+  if (S < 4 && 5 * per_site <= 65536) S = 4;
   e->kbegin(K_PRODUCTS);
   if ((size_t)(S + 1) * per_site <= 65536) {
     const unsigned nb = (unsigned)((e->n_sites + S - 1) / S);
-    hipLaunchKernelGGL(k_products, dim3(nb), dim3(NT), (size_t)(S + 1) * per_site, e->stream, e->coeffs,
-                       e->n_sites, e->n_states, e->n_orb, S, e->rho);
+    if (e->n_states <= 8)
+      hipLaunchKernelGGL(k_products<false>, dim3(nb), dim3(NT), (size_t)(S + 1) * per_site, e->stream, e->coeffs,
+                         e->n_sites, e->n_states, e->n_orb, S, e->rho);
+    else
+      hipLaunchKernelGGL(k_products<true>, dim3(nb), dim3(NT), (size_t)(S + 1) * per_site, e->stream, e->coeffs,
+                         e->n_sites, e->n_states, e->n_orb, S, e->rho);
   } else {
     hipLaunchKernelGGL(k_products_direct, dim3(grid1d((size_t)P * e->n_sites)), dim3(256), 0, e->stream,
                        e->coeffs, e->n_sites, e->n_states, e->n_orb, e->rho);
