@@ -271,6 +271,9 @@ struct clat_engine {
   double2* b_out2[LINE_NB] = {};
   bool pack = true;                     // pairs of real (diagonal) densities share a prepare
   int n_cu = 256;     // compute units of the device
+  // host buffers of a run, kept with their pages between runs
+  std::vector<host::PlanStep> plan_steps;
+  std::vector<double> ov_host;
   int ovl_mode = 0;  // overlap kernel: 0 by size, 1 vector FMA, 2 FP64 MFMA, 3 vector FMA in 48 x 48 tiles
   bool table_pending = false;           // stage_ms[0] of the last table build not read yet
   size_t a_set = 0, b_set = 0, d_set = 0;  // elements of A, B, D per prepare
@@ -1662,7 +1665,8 @@ int clat_compute_integrals(clat_engine* e, const int32_t* quads, int64_t n, int 
     // the stale-values bookkeeping walks the steps in execution order; without
     // it every value depends on its own step only and id order serves
     const bool stale = (flags & CLAT_FLAG_STALE_VALUES) != 0;
-    std::vector<host::PlanStep> steps = host::make_plan(quads, n, e->n_states, stale);
+    host::make_plan_into(quads, n, e->n_states, stale, e->plan_steps);
+    const std::vector<host::PlanStep>& steps = e->plan_steps;
     const int P = e->n_states * (e->n_states + 1) / 2;
     // left products in execution order (ascending product index) -> slots
     std::vector<int> slot_of((size_t)P, -1), slots;
@@ -1747,7 +1751,8 @@ int clat_compute_integrals(clat_engine* e, const int32_t* quads, int64_t n, int 
       e->ovl = (double*)e->dalloc(n_out * sizeof(double));
       e->ovl_cap = n_out;
     }
-    std::vector<double> ov(n_out, 0.0);
+    std::vector<double>& ov = e->ov_host;  // kept between runs, like the plan buffer
+    ov.resize(n_out);
     if (nL > 0 && e->n_sites > 0) {
       e->kbegin(K_OVERLAP);
       const dim3 og(tl, tr, (unsigned)n_chunks);

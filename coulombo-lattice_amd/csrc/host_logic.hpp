@@ -309,9 +309,12 @@ inline size_t parallel_ranges(size_t n, F&& fn) {
   return nt;
 }
 
-inline std::vector<PlanStep> make_plan(const int32_t* quads, int64_t n, int n_states, bool ordered = true) {
+// The plan into `steps`, whose capacity is reused: a caller that plans
+// again and again keeps one buffer and its pages.
+inline void make_plan_into(const int32_t* quads, int64_t n, int n_states, bool ordered,
+                           std::vector<PlanStep>& steps) {
   if (n > std::numeric_limits<int>::max()) throw Error(1, "too many integrals to compute");
-  std::vector<PlanStep> steps((size_t)n);
+  steps.resize((size_t)n);
   const int64_t P = (int64_t)n_states * (n_states + 1) / 2;
   std::vector<char> bad(16, 0);  // a range check failed in thread t
   parallel_ranges((size_t)n, [&](size_t k0, size_t k1, size_t t) {
@@ -372,7 +375,7 @@ inline std::vector<PlanStep> make_plan(const int32_t* quads, int64_t n, int n_st
     for (size_t k = k0; k < k1; ++k)
       if (!in_cover[(size_t)steps[k].left.product]) std::swap(steps[k].left, steps[k].right);
   });
-  if (!ordered) return steps;
+  if (!ordered) return;
   // order (Planner.hpp:94-101): left product, right product, then the steps
   // whose two products have equal conj flags before the others
   auto rc = [](const PlanStep& s) { return (s.left.conj != s.right.conj) ? 1 : 0; };
@@ -390,7 +393,8 @@ inline std::vector<PlanStep> make_plan(const int32_t* quads, int64_t n, int n_st
     for (int64_t i = 0; i < n_keys; ++i) start[(size_t)i + 1] += start[(size_t)i];
     std::vector<PlanStep> sorted(steps.size());
     for (size_t k = 0; k < steps.size(); ++k) sorted[start[key[k]]++] = steps[k];
-    return sorted;
+    std::copy(sorted.begin(), sorted.end(), steps.begin());
+    return;
   }
   if (n_keys < ((int64_t)1 << 32)) {
     std::vector<uint64_t> key;
@@ -400,13 +404,19 @@ inline std::vector<PlanStep> make_plan(const int32_t* quads, int64_t n, int n_st
     radix_sort_u64(key, 64);
     std::vector<PlanStep> sorted(steps.size());
     for (size_t i = 0; i < key.size(); ++i) sorted[i] = steps[(size_t)(key[i] & 0x7FFFFFFF)];
-    return sorted;
+    std::copy(sorted.begin(), sorted.end(), steps.begin());
+    return;
   }
   std::sort(steps.begin(), steps.end(), [&rc](const PlanStep& a, const PlanStep& b) {
     if (a.left.product != b.left.product) return a.left.product < b.left.product;
     if (a.right.product != b.right.product) return a.right.product < b.right.product;
     return rc(a) < rc(b);
   });
+}
+
+inline std::vector<PlanStep> make_plan(const int32_t* quads, int64_t n, int n_states, bool ordered = true) {
+  std::vector<PlanStep> steps;
+  make_plan_into(quads, n, n_states, ordered, steps);
   return steps;
 }
 
