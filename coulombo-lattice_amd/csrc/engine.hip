@@ -9,7 +9,6 @@
 #include <cstring>
 #include <array>
 #include <string>
-#include <thread>
 #include <vector>
 
 #include "../../include/clat.h"
@@ -1588,7 +1587,7 @@ int clat_set_atoms(clat_engine* e, const int32_t* cell, int64_t n_atoms, int n_o
         mask[(size_t)(z * yr1 + y % yr1)] |= 1u << (y / yr1);
       }
     }
-    free_atoms(e);
+    free_atoms(e);  // also drops the loaded states (stored in the old site order)
     e->n_atoms = n_atoms;
     e->n_orb = n_orbitals;
     e->n_sites = (int64_t)site_atom.size();
@@ -1753,6 +1752,8 @@ int clat_compute_integrals(clat_engine* e, const int32_t* quads, int64_t n, int 
     }
     std::vector<double>& ov = e->ov_host;  // kept between runs, like the plan buffer
     ov.resize(n_out);
+    // with no sites every overlap is zero (the buffer holds the last run's)
+    if (!(nL > 0 && e->n_sites > 0)) std::fill(ov.begin(), ov.end(), 0.0);
     if (nL > 0 && e->n_sites > 0) {
       e->kbegin(K_OVERLAP);
       const dim3 og(tl, tr, (unsigned)n_chunks);
@@ -1783,8 +1784,8 @@ int clat_compute_integrals(clat_engine* e, const int32_t* quads, int64_t n, int 
       else { vre = Av - Bv; vim = Cv + Dv; }
     };
     if (!stale) {
-      // every value from its own step: up to 16 host threads over id ranges
-      auto run = [&](size_t k0, size_t k1) {
+      // every value from its own step: host threads over id ranges
+      host::parallel_ranges(steps.size(), [&](size_t k0, size_t k1, size_t) {
         for (size_t k = k0; k < k1; ++k) {
           const auto& s = steps[k];
           double vre, vim;
@@ -1792,18 +1793,7 @@ int clat_compute_integrals(clat_engine* e, const int32_t* quads, int64_t n, int 
           out[2 * (size_t)s.id] = vre;
           out[2 * (size_t)s.id + 1] = s.left.conj ? -vim : vim;
         }
-      };
-      const size_t ns = steps.size();
-      const size_t nt = std::min<size_t>({(size_t)16, (size_t)std::max(1u, std::thread::hardware_concurrency()),
-                                          std::max<size_t>(1, ns / 65536)});
-      if (nt <= 1) {
-        run(0, ns);
-      } else {
-        std::vector<std::thread> pool;
-        for (size_t t = 1; t < nt; ++t) pool.emplace_back(run, ns * t / nt, ns * (t + 1) / nt);
-        run(0, ns / nt);
-        for (auto& th : pool) th.join();
-      }
+      });
     } else {
       int lastR = -1, lastRc = 0;
       double vre = 0.0, vim = 0.0;

@@ -148,7 +148,8 @@ int clat_get_potential(clat_engine* e, double* V);
 
 /* Atom -> cell map of the lattice (Broadcaster.cpp:76-88) and the orbital
  * count (coulombo.cpp:59-65).  Where several atoms share a cell, the last
- * one wins, as Product::map's assignment does (Product.cpp:22,27). */
+ * one wins, as Product::map's assignment does (Product.cpp:22,27).  States
+ * loaded before the call are dropped (they belong to the old atom list). */
 int clat_set_atoms(clat_engine* e, const int32_t* cell, int64_t n_atoms,
                    int n_orbitals);
 

@@ -228,6 +228,33 @@ def test_atoms_sharing_cells(prec, sched):
     assert np.max(np.abs(v - Vo)) / np.max(np.abs(Vo)) < TOL[prec]
 
 
+def test_atoms_set_again():
+    """A second clat_set_atoms drops the loaded states; with no atoms every
+    integral is zero, also on an engine whose previous run had values."""
+    print("case atoms set again", flush=True)
+    w = zb.workload(2, 2, 2, 1, 1, config=0, sigma_frac=0.2, n_orb=4)
+    X, Y, Z = w["dims"]
+    eng = clat.Engine((X, Y, Z), clat.PREC_FP32X, clat.SCHED_SHARED)
+    eng.initialize(clat.InteractionSimple(w["step"], 0.0, 10.89))
+    eng.set_atoms(w["cells"], 4)
+    for s in w["states"]:
+        eng.add_state(s)
+    quads = clat.pattern_enumerate("****", 2, 1)
+    first = eng.compute_integrals(quads)
+    assert np.max(np.abs(first)) > 0
+    eng.set_atoms(np.zeros(0, np.int32), 4)
+    with pytest.raises(clat.ClatError):
+        eng.compute_integrals(quads)  # no states loaded
+    for _ in range(2):
+        eng.add_state(np.zeros((0, 4), complex))
+    assert np.all(eng.compute_integrals(quads) == 0)
+    # and back to the first atom list: the first values again
+    eng.set_atoms(w["cells"], 4)
+    for s in w["states"]:
+        eng.add_state(s)
+    assert np.array_equal(eng.compute_integrals(quads), first)
+
+
 def test_empty_and_invalid_integral_lists():
     """No integrals (an empty list, or a pattern that selects none: the
     reference then writes no files) gives an empty result; a state index past
