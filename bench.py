@@ -300,6 +300,18 @@ def main():
         "fft_pass_gbs": {k: (v[2] / max(1e-12, v[1] * 1e-3)) / 1e9 for k, v in head["breakdown"].items()
                          if v[0] and k in ("xfwd", "yfwd", "zconv", "yinv", "xinv")},
     }
+    # FFT HBM GB/s: measured HBM bytes per launch of each pass (the committed
+    # FETCH_SIZE + WRITE_SIZE summary of this workload) times launches per
+    # step, over the five passes' event-timed duration per step
+    fft_kinds = ("xfwd", "yfwd", "zconv", "yinv", "xinv")
+    fft_tr = {k: traffic_of(k, default_workload) for k in fft_kinds}
+    if all(fft_tr[k] for k in fft_kinds) and all(head["breakdown"][k][0] for k in fft_kinds):
+        b = head["breakdown"]
+        hbm = sum(fft_tr[k] * b[k][0] for k in fft_kinds)
+        t_s = sum(b[k][1] for k in fft_kinds) * 1e-3
+        out["fft_hbm_gbs"] = {"value": hbm / t_s / 1e9, "peak": PEAK_HBM_GBS,
+                              "frac": hbm / t_s / 1e9 / PEAK_HBM_GBS,
+                              "bytes_per_step": hbm, "pass_ms_per_step": t_s * 1e3}
     if "fp64" in res:
         v64 = res["fp64"]["vals"]
         for name in ("fp32x", "fp32", "fp64_rounds"):
