@@ -8,6 +8,8 @@ enough for it:
 * multi-system: boxes of 16^3 ... 100^3 cells (grids 64^3 ... 400^3, the
   last 8M atoms), 2 states each.
 
+In the batched form three integrals are also checked against the oracle's
+eight-round prepare on the full 128^3 grid.
 Properties: the kernel G is real and even, so
   I(i1,i2,i3,i4) = I(i2,i1,i4,i3)          (both densities swapped)
   I(i1,i2,i3,i4) = conj(I(i4,i3,i2,i1))    (Hermitian)
@@ -73,6 +75,23 @@ def test_batched_form_full_size():
     _check_symmetries(quads, vals[clat.PREC_FP64], 1e-12)
     assert _rel_err(vals[clat.PREC_FP32X], vals[clat.PREC_FP64]) < 1e-5
     assert np.all(np.isfinite(vals[clat.PREC_FP32X][sub]))
+    # three integrals against the oracle's eight-round prepare on the same
+    # 128^3 grid, given only the pair densities they use (p = L(L-1)/2 + R - 1)
+    X, Y, Z = w["dims"]
+    st = w["states"]
+    pick = [(1, 1, 1, 1), (1, 2, 3, 4), (20, 3, 9, 31)]
+    rho = {}
+    for (a, b, c, d) in pick:
+        for (i, j) in ((a, d), (b, c)):
+            hi, lo = max(i, j), min(i, j)
+            pair = np.stack([st[hi - 1]]) if hi == lo else np.stack([st[lo - 1], st[hi - 1]])
+            rho[hi * (hi - 1) // 2 + lo - 1] = O.products(pair)[0 if hi == lo else 1]
+    Gh = O.ghat(O.gtable(X, Y, Z, w["step"], 0.0, 10.89))
+    want = O.compute_integrals(pick, 32, rho, w["cells"], (X, Y, Z), Gh)
+    at = _index(quads)
+    for prec, tol in ((clat.PREC_FP64, 1e-10), (clat.PREC_FP32X, 1e-5)):
+        got = np.array([vals[prec][at[q]] for q in pick])
+        assert _rel_err(got, want) < tol, (prec, got, want)
 
 
 @pytest.mark.parametrize("cells", [(16, 16, 16), (24, 24, 24), (30, 30, 30), (40, 40, 40), (48, 48, 48),
