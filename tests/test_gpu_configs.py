@@ -94,6 +94,42 @@ def test_batched_form_full_size():
         assert _rel_err(got, want) < tol, (prec, got, want)
 
 
+def test_one_million_atoms_full_size():
+    """The 1M-atom workload as the bench runs it: 64x64x32 cells = 1,048,576
+    atoms on a 256x256x128 grid, 8 states (4 holes, 4 electrons), 20 orbitals,
+    eeee,hhhh,ehhe,eheh (1024 integrals).  FP64 symmetries, FP32X within 1e-5
+    of FP64, and one integral of each kind against the oracle's eight-round
+    prepare on the full grid."""
+    print("case config 1M atoms 256x256x128, 8 states", flush=True)
+    w = zb.workload(64, 64, 32, 4, 4, config=1, sigma_frac=0.15)
+    X, Y, Z = w["dims"]
+    assert (X, Y, Z) == (256, 256, 128) and len(w["cells"]) == 1048576
+    quads = clat.pattern_enumerate("eeee,hhhh,ehhe,eheh", 8, 4)
+    assert len(quads) == 1024
+    vals = {}
+    for prec in (clat.PREC_FP32X, clat.PREC_FP64):
+        eng = _engine(w, prec, 20)
+        vals[prec] = eng.compute_integrals(quads)
+        eng.close()
+    _check_symmetries(quads, vals[clat.PREC_FP64], 1e-12)
+    assert _rel_err(vals[clat.PREC_FP32X], vals[clat.PREC_FP64]) < 1e-5
+    st = w["states"]
+    pick = [(5, 5, 5, 5), (1, 2, 3, 4), (6, 2, 2, 7), (6, 2, 7, 3)]
+    at = _index(quads)
+    assert all(q in at for q in pick)
+    rho = {}
+    for (a, b, c, d) in pick:
+        for (i, j) in ((a, d), (b, c)):
+            hi, lo = max(i, j), min(i, j)
+            pair = np.stack([st[hi - 1]]) if hi == lo else np.stack([st[lo - 1], st[hi - 1]])
+            rho[hi * (hi - 1) // 2 + lo - 1] = O.products(pair)[0 if hi == lo else 1]
+    Gh = O.ghat(O.gtable(X, Y, Z, w["step"], 0.0, 10.89))
+    want = O.compute_integrals(pick, 8, rho, w["cells"], (X, Y, Z), Gh)
+    for prec, tol in ((clat.PREC_FP64, 1e-10), (clat.PREC_FP32X, 1e-5)):
+        got = np.array([vals[prec][at[q]] for q in pick])
+        assert _rel_err(got, want) < tol, (prec, got, want)
+
+
 @pytest.mark.parametrize("cells", [(16, 16, 16), (24, 24, 24), (30, 30, 30), (40, 40, 40), (48, 48, 48),
                                    (100, 100, 100)])
 def test_multi_system_sizes(cells):
