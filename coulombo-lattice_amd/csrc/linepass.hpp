@@ -111,9 +111,13 @@ __global__ __launch_bounds__((line_threads<C, R1, R2, NSUB>())) void k_linepass(
   const bool ok2 = tid < C * NSUB * R2 && c2 < nC;
 
   if constexpr (MODE == PM_FWD) {
-    // pass twiddles W_2L^m, m < 2L, in LDS (the scatter's barriers cover the fill)
-    __shared__ CT twl[2 * L];
-    for (int i = tid; i < 2 * L; i += NTH) twl[i] = T2L[i];
+    // FP32: pass twiddles W_2L^m, m < 2L, in LDS (the scatter's barriers
+    // cover the fill).  FP64: none in LDS (it would cost a block per CU);
+    // each thread forms its R2 twiddles from two table entries, below.
+    constexpr bool TWL = sizeof(TC) == 4;
+    __shared__ CT twl[TWL ? 2 * L : 1];
+    if constexpr (TWL)
+      for (int i = tid; i < 2 * L; i += NTH) twl[i] = T2L[i];
     auto TW = [&](uint32_t byte_off) {
       return *reinterpret_cast<const CT*>(reinterpret_cast<const char*>(twl) + byte_off);
     };
@@ -135,12 +139,26 @@ __global__ __launch_bounds__((line_threads<C, R1, R2, NSUB>())) void k_linepass(
     if (ok13) {
       if (s13 == 1) rot_all<R2, false, TC>(v);
       rdft<R2, false, TC>(v);
-      const uint32_t tstep = (uint32_t)(2 * n1) * SC, tbase = (uint32_t)(s13 * n1) * SC;
+      if constexpr (TWL) {
+        const uint32_t tstep = (uint32_t)(2 * n1) * SC, tbase = (uint32_t)(s13 * n1) * SC;
 #pragma unroll
-      for (int q = 0; q < R2; ++q) {
-        CT t = v[q];
-        if (q > 0 || s13 == 1) t = cmul<TC>(t, TW(tbase + (uint32_t)q * tstep));
-        lds[xi(c13, s13, q, n1)] = t;
+        for (int q = 0; q < R2; ++q) {
+          CT t = v[q];
+          if (q > 0 || s13 == 1) t = cmul<TC>(t, TW(tbase + (uint32_t)q * tstep));
+          lds[xi(c13, s13, q, n1)] = t;
+        }
+      } else {
+        // W_2L^{n1 (2q + s13)} = W^{s13 n1} (W^{2 n1})^q, the powers by
+        // repeated products (at most R2 - 1 deep: ~1e-15 relative in FP64)
+        const CT d = T2L[2 * n1];
+        CT cur = T2L[s13 * n1];
+#pragma unroll
+        for (int q = 0; q < R2; ++q) {
+          CT t = v[q];
+          if (q > 0 || s13 == 1) t = cmul<TC>(t, cur);
+          lds[xi(c13, s13, q, n1)] = t;
+          cur = cmul<TC>(cur, d);
+        }
       }
     }
     __syncthreads();
