@@ -272,6 +272,7 @@ struct clat_engine {
   int n_cu = 256;     // compute units of the device
   // host buffers of a run, kept with their pages between runs
   std::vector<host::PlanStep> plan_steps;
+  std::vector<char> plan_in_cover, plan_leads;
   std::vector<double> ov_host;
   int ovl_mode = 0;  // overlap kernel: 0 by size, 1 vector FMA, 2 FP64 MFMA, 3 vector FMA in 48 x 48 tiles
   bool table_pending = false;           // stage_ms[0] of the last table build not read yet
@@ -1664,14 +1665,14 @@ int clat_compute_integrals(clat_engine* e, const int32_t* quads, int64_t n, int 
     // the stale-values bookkeeping walks the steps in execution order; without
     // it every value depends on its own step only and id order serves
     const bool stale = (flags & CLAT_FLAG_STALE_VALUES) != 0;
-    host::make_plan_into(quads, n, e->n_states, stale, e->plan_steps);
-    const std::vector<host::PlanStep>& steps = e->plan_steps;
+    // the planner's cover fixes which products lead steps and so the prepare
+    // order; the steps themselves are built while the prepares run
+    host::plan_cover(quads, n, e->n_states, e->plan_in_cover, e->plan_leads);
     const int P = e->n_states * (e->n_states + 1) / 2;
     // left products in execution order (ascending product index) -> slots
     std::vector<int> slot_of((size_t)P, -1), slots;
-    for (const auto& s : steps) slot_of[(size_t)s.left.product] = 0;
     for (int p = 0; p < P; ++p)
-      if (slot_of[(size_t)p] == 0) {
+      if (e->plan_leads[(size_t)p]) {
         slot_of[(size_t)p] = (int)slots.size();
         slots.push_back(p);
       }
@@ -1771,9 +1772,14 @@ int clat_compute_integrals(clat_engine* e, const int32_t* quads, int64_t n, int 
                          n_slices, (int64_t)n_out, e->ovl);
       LAUNCH_CHECK();
       e->kend(K_OVERLAP, (double)(tl * tr) * (double)(2 * OV_T) * (double)e->n_sites * 16.0);
-      HIPX(hipMemcpyAsync(ov.data(), e->ovl, n_out * sizeof(double), hipMemcpyDeviceToHost, e->stream));
     }
     HIPX(hipEventRecord(e->stage_ev[4], e->stream));
+    // the steps, on the host while the queued prepares and overlaps run (the
+    // copy of the overlap table to pageable memory below waits for them)
+    host::make_plan_into(quads, n, e->n_states, stale, e->plan_steps, &e->plan_in_cover);
+    const std::vector<host::PlanStep>& steps = e->plan_steps;
+    if (nL > 0 && e->n_sites > 0)
+      HIPX(hipMemcpyAsync(ov.data(), e->ovl, n_out * sizeof(double), hipMemcpyDeviceToHost, e->stream));
     HIPX(hipStreamSynchronize(e->stream));
     e->kcollect();
     // main loop bookkeeping (coulombo.cpp:186-209)

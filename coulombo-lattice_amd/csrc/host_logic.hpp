@@ -309,51 +309,32 @@ inline size_t parallel_ranges(size_t n, F&& fn) {
   return nt;
 }
 
-// The plan into `steps`, whose capacity is reused: a caller that plans
-// again and again keeps one buffer and its pages.
-inline void make_plan_into(const int32_t* quads, int64_t n, int n_states, bool ordered,
-                           std::vector<PlanStep>& steps) {
-  if (n > std::numeric_limits<int>::max()) throw Error(1, "too many integrals to compute");
-  steps.resize((size_t)n);
-  const int64_t P = (int64_t)n_states * (n_states + 1) / 2;
-  std::vector<char> bad(16, 0);  // a range check failed in thread t
-  parallel_ranges((size_t)n, [&](size_t k0, size_t k1, size_t t) {
-    for (size_t k = k0; k < k1; ++k) {
-      const int32_t* q = quads + 4 * k;
-      for (int j = 0; j < 4; ++j)
-        if (q[j] < 1 || q[j] > n_states) {
-          bad[t] = 1;
-          return;
-        }
-      steps[k] = PlanStep{(int)k, plan_product(q[0], q[3]), plan_product(q[1], q[2])};
-    }
-  });
-  for (char b : bad)
-    if (b) throw Error(1, "integral state index out of range");
-  // the product graph's edges, sorted and unique: a bitmap over P x P when
-  // it is small, else a sort of the edge keys
+// The product graph's edges (left * P + right of every integral), sorted and
+// unique: a bitmap over P x P when it is small (per-thread bitmaps OR-merged
+// while they stay small), else a sort of the edge keys.
+template <typename KeyOf>
+inline std::vector<std::pair<int, int>> product_edges(size_t n, int64_t P, KeyOf key_of) {
   std::vector<std::pair<int, int>> edges;
   if (P * P <= ((int64_t)1 << 28)) {
     const size_t words = (size_t)((P * P + 63) / 64);
     std::vector<uint64_t> bits(words, 0);
-    // per-thread bitmaps, OR-merged, while they stay small
     const size_t nt_max = words <= ((size_t)1 << 16) ? 16 : 1;
     std::vector<std::vector<uint64_t>> part(nt_max > 1 ? nt_max : 0);
     auto mark = [&](size_t k0, size_t k1, uint64_t* b) {
       for (size_t k = k0; k < k1; ++k) {
-        const uint64_t e = (uint64_t)steps[k].left.product * P + steps[k].right.product;
+        const uint64_t e = key_of(k);
         b[e >> 6] |= (uint64_t)1 << (e & 63);
       }
     };
     if (nt_max > 1) {
-      const size_t nt = parallel_ranges(steps.size(), [&](size_t k0, size_t k1, size_t t) {
+      const size_t nt = parallel_ranges(n, [&](size_t k0, size_t k1, size_t t) {
         part[t].assign(words, 0);
         mark(k0, k1, part[t].data());
       });
       for (size_t t = 0; t < nt; ++t)
         for (size_t w = 0; w < words; ++w) bits[w] |= part[t][w];
     } else {
-      mark(0, steps.size(), bits.data());
+      mark(0, n, bits.data());
     }
     for (size_t w = 0; w < bits.size(); ++w)
       for (uint64_t b = bits[w]; b; b &= b - 1) {
@@ -361,16 +342,72 @@ inline void make_plan_into(const int32_t* quads, int64_t n, int n_states, bool o
         edges.push_back({(int)(e / P), (int)(e % P)});
       }
   } else {
-    std::vector<uint64_t> ek;
-    ek.reserve(steps.size());
-    for (const PlanStep& s : steps) ek.push_back((uint64_t)s.left.product * P + s.right.product);
+    std::vector<uint64_t> ek(n);
+    for (size_t k = 0; k < n; ++k) ek[k] = key_of(k);
     std::sort(ek.begin(), ek.end());
     ek.erase(std::unique(ek.begin(), ek.end()), ek.end());
     for (uint64_t e : ek) edges.push_back({(int)(e / P), (int)(e % P)});
   }
-  const std::vector<int> cv = vertex_cover((int)P, edges);
-  std::vector<char> in_cover((size_t)P, 0);
-  for (int v : cv) in_cover[(size_t)v] = 1;
+  return edges;
+}
+
+// Every state index in 1..n_states (checked over all integrals in parallel).
+inline void check_quads(const int32_t* quads, int64_t n, int n_states) {
+  if (n > std::numeric_limits<int>::max()) throw Error(1, "too many integrals to compute");
+  std::vector<char> bad(16, 0);  // a range check failed in thread t
+  parallel_ranges((size_t)n, [&](size_t k0, size_t k1, size_t t) {
+    for (size_t k = 4 * k0; k < 4 * k1; ++k)
+      if (quads[k] < 1 || quads[k] > n_states) {
+        bad[t] = 1;
+        return;
+      }
+  });
+  for (char b : bad)
+    if (b) throw Error(1, "integral state index out of range");
+}
+
+// The vertex cover of the product graph (in_cover[p]) and the products that
+// lead at least one step once the steps are swapped into the cover (leads[p]),
+// from the integrals alone: what orders the prepares, ahead of building the
+// steps themselves.
+inline void plan_cover(const int32_t* quads, int64_t n, int n_states, std::vector<char>& in_cover,
+                       std::vector<char>& leads) {
+  check_quads(quads, n, n_states);
+  const int64_t P = (int64_t)n_states * (n_states + 1) / 2;
+  const auto edges = product_edges((size_t)n, P, [&](size_t k) {
+    const int32_t* q = quads + 4 * k;
+    return (uint64_t)plan_product(q[0], q[3]).product * P + plan_product(q[1], q[2]).product;
+  });
+  in_cover.assign((size_t)P, 0);
+  for (int v : vertex_cover((int)P, edges)) in_cover[(size_t)v] = 1;
+  leads.assign((size_t)P, 0);
+  for (const auto& e : edges) leads[(size_t)(in_cover[(size_t)e.first] ? e.first : e.second)] = 1;
+}
+
+// The plan into `steps`, whose capacity is reused: a caller that plans
+// again and again keeps one buffer and its pages.  `cover`, when given, is
+// plan_cover's in_cover for the same integrals (checked there).
+inline void make_plan_into(const int32_t* quads, int64_t n, int n_states, bool ordered,
+                           std::vector<PlanStep>& steps, const std::vector<char>* cover = nullptr) {
+  if (!cover) check_quads(quads, n, n_states);
+  steps.resize((size_t)n);
+  const int64_t P = (int64_t)n_states * (n_states + 1) / 2;
+  parallel_ranges((size_t)n, [&](size_t k0, size_t k1, size_t) {
+    for (size_t k = k0; k < k1; ++k) {
+      const int32_t* q = quads + 4 * k;
+      steps[k] = PlanStep{(int)k, plan_product(q[0], q[3]), plan_product(q[1], q[2])};
+    }
+  });
+  std::vector<char> own;
+  if (!cover) {
+    const auto edges = product_edges(steps.size(), P, [&](size_t k) {
+      return (uint64_t)steps[k].left.product * P + steps[k].right.product;
+    });
+    own.assign((size_t)P, 0);
+    for (int v : vertex_cover((int)P, edges)) own[(size_t)v] = 1;
+    cover = &own;
+  }
+  const std::vector<char>& in_cover = *cover;
   parallel_ranges(steps.size(), [&](size_t k0, size_t k1, size_t) {
     for (size_t k = k0; k < k1; ++k)
       if (!in_cover[(size_t)steps[k].left.product]) std::swap(steps[k].left, steps[k].right);
