@@ -5,6 +5,7 @@
 #pragma once
 
 #include <algorithm>
+#include <atomic>
 #include <cctype>
 #include <cmath>
 #include <cstdint>
@@ -372,12 +373,21 @@ inline void check_quads(const int32_t* quads, int64_t n, int n_states) {
 // steps themselves.
 inline void plan_cover(const int32_t* quads, int64_t n, int n_states, std::vector<char>& in_cover,
                        std::vector<char>& leads) {
-  check_quads(quads, n, n_states);
+  if (n > std::numeric_limits<int>::max()) throw Error(1, "too many integrals to compute");
   const int64_t P = (int64_t)n_states * (n_states + 1) / 2;
-  const auto edges = product_edges((size_t)n, P, [&](size_t k) {
+  // the range check rides on the edge scan (one pass over the integrals); an
+  // out-of-range integral sets the flag and contributes edge 0
+  std::atomic<bool> bad{false};
+  const auto edges = product_edges((size_t)n, P, [&](size_t k) -> uint64_t {
     const int32_t* q = quads + 4 * k;
+    for (int j = 0; j < 4; ++j)
+      if (q[j] < 1 || q[j] > n_states) {
+        bad.store(true, std::memory_order_relaxed);
+        return 0;
+      }
     return (uint64_t)plan_product(q[0], q[3]).product * P + plan_product(q[1], q[2]).product;
   });
+  if (bad.load()) throw Error(1, "integral state index out of range");
   in_cover.assign((size_t)P, 0);
   for (int v : vertex_cover((int)P, edges)) in_cover[(size_t)v] = 1;
   leads.assign((size_t)P, 0);
