@@ -76,7 +76,10 @@ def counters(d, counter):
 
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--trace", required=True, help="output dir of the --kernel-trace --stats run")
+    ap.add_argument("--trace", help="output dir of the --kernel-trace --stats run")
+    ap.add_argument("--sq", nargs="*", default=[],
+                    help="output dirs of SQ counter runs (SQ_WAVES, SQ_WAVE_CYCLES, SQ_WAIT_ANY, "
+                         "SQ_INSTS_VALU, SQ_LDS_BANK_CONFLICT, SQ_LDS_IDX_ACTIVE, ...)")
     ap.add_argument("--fetch", help="output dir of the FETCH_SIZE run")
     ap.add_argument("--write", help="output dir of the WRITE_SIZE run")
     ap.add_argument("--copy-fetch", help="FETCH_SIZE run of bench.py --copy-probe")
@@ -89,13 +92,37 @@ def main():
     ap.add_argument("--traffic-json")
     a = ap.parse_args()
 
-    stats = trace_stats(a.trace)
-    total = sum(t for _, _, t, _ in stats)
-    out = ["# %s" % a.title, "", "Commands:", "", "```", a.commands.strip(), "```", "",
-           "| kernel | kind | calls | avg (us) | total (ms) | share |", "|---|---|---:|---:|---:|---:|"]
-    for n, c, t, avg in stats:
-        out.append("| `%s` | %s | %d | %.1f | %.2f | %.1f%% |" % (short(n), kind_of(n), c, avg, t / 1e3,
-                                                                   100.0 * t / total))
+    out = ["# %s" % a.title, "", "Commands:", "", "```", a.commands.strip(), "```", ""]
+    if a.trace:
+        stats = trace_stats(a.trace)
+        total = sum(t for _, _, t, _ in stats)
+        out += ["| kernel | kind | calls | avg (us) | total (ms) | share |", "|---|---|---:|---:|---:|---:|"]
+        for n, c, t, avg in stats:
+            out.append("| `%s` | %s | %d | %.1f | %.2f | %.1f%% |" % (short(n), kind_of(n), c, avg, t / 1e3,
+                                                                       100.0 * t / total))
+    if a.sq:
+        # per-dispatch averages of the SQ counters, merged over the runs
+        cs = {}
+        for d in a.sq:
+            con = sqlite3.connect(db_of(d))
+            for c in {r[0] for r in con.execute("select distinct counter_name from counters_collection")}:
+                for k, v in counters(d, c).items():
+                    cs.setdefault(k, {})[c] = v
+        out += ["SQ counters per dispatch (averages over the run's dispatches). Waiting share = SQ_WAIT_ANY / "
+                "SQ_WAVE_CYCLES (wave-cycles waiting on anything); LDS bank-conflict share = "
+                "SQ_LDS_BANK_CONFLICT / SQ_LDS_IDX_ACTIVE.", "",
+                "| kernel | waves | VALU instr per wave | LDS instr per wave | waiting share | LDS bank-conflict share |",
+                "|---|---:|---:|---:|---:|---:|"]
+        for k in sorted(cs, key=lambda n: kind_of(n)):
+            v = cs[k]
+            if "SQ_WAVES" not in v or not v["SQ_WAVES"]:
+                continue
+            w = v["SQ_WAVES"]
+            out.append("| `%s` | %.0f | %.0f | %s | %s | %s |" % (
+                short(k), w, v.get("SQ_INSTS_VALU", 0) / w,
+                "%.0f" % (v["SQ_INSTS_LDS"] / w) if "SQ_INSTS_LDS" in v else "—",
+                "%.2f" % (v["SQ_WAIT_ANY"] / v["SQ_WAVE_CYCLES"]) if v.get("SQ_WAVE_CYCLES") else "—",
+                "%.2f" % (v["SQ_LDS_BANK_CONFLICT"] / v["SQ_LDS_IDX_ACTIVE"]) if v.get("SQ_LDS_IDX_ACTIVE") else "—"))
     traffic = None
     if a.fetch and a.write and a.copy_fetch and a.copy_write:
         copy_bytes = a.copy_mib * (1 << 20)
