@@ -44,19 +44,59 @@ def cells_of(idx: np.ndarray, dims) -> np.ndarray:
     return ((idx[:, 2] * Y + idx[:, 1]) * X + idx[:, 0]).astype(np.int32)
 
 
+BLOCK = 1 << 20          # atoms per independently seeded block (large boxes)
+BIG = 2 * BLOCK          # boxes above this many atoms draw their normals in blocks
+
+
+def _envelope(pos, r0, sig, hole):
+    d = (pos - r0) / sig
+    env = np.exp(-0.5 * np.sum(d * d, axis=1))
+    return env * d[:, 0] if hole else env
+
+
 def state(pos: np.ndarray, box, hole: bool, seed: int, sigma_frac: float, n_orb: int = 20) -> np.ndarray:
+    """One state's coefficients, shape (n_atoms, n_orb), complex128.
+
+    Up to BIG atoms one PCG64 stream (seed) draws the centre offset, then all
+    real parts, then all imaginary parts.  Larger boxes (the 8M-atom grid)
+    draw the offset from that stream and the normals in blocks of BLOCK atoms,
+    block b from PCG64 seeded with (seed, b), on parallel threads; the values
+    do not depend on the thread count."""
     rng = np.random.default_rng(seed)
     box = np.asarray(box, np.float64)
     r0 = box / 2 + rng.uniform(-0.05, 0.05, 3) * box
     sig = sigma_frac * box
-    d = (pos - r0) / sig
-    env = np.exp(-0.5 * np.sum(d * d, axis=1))
-    if hole:
-        env = env * d[:, 0]
-    re = rng.standard_normal((len(pos), n_orb))
-    im = rng.standard_normal((len(pos), n_orb))
-    c = (re + 1j * im) * env[:, None]
-    c /= np.sqrt(np.sum(np.abs(c) ** 2))
+    n = len(pos)
+    if n <= BIG:
+        env = _envelope(pos, r0, sig, hole)
+        re = rng.standard_normal((n, n_orb))
+        im = rng.standard_normal((n, n_orb))
+        c = (re + 1j * im) * env[:, None]
+        c /= np.sqrt(np.sum(np.abs(c) ** 2))
+        return c
+    from concurrent.futures import ThreadPoolExecutor
+    c = np.empty((n, n_orb), np.complex128)
+    nblk = (n + BLOCK - 1) // BLOCK
+
+    def fill(b):
+        lo, hi = b * BLOCK, min(n, (b + 1) * BLOCK)
+        g = np.random.default_rng([seed, b])
+        v = c[lo:hi]
+        v.real = g.standard_normal((hi - lo, n_orb))
+        v.imag = g.standard_normal((hi - lo, n_orb))
+        v *= _envelope(pos[lo:hi], r0, sig, hole)[:, None]
+        return float(np.sum(v.real ** 2) + np.sum(v.imag ** 2))
+
+    workers = max(1, min(16, os.cpu_count() or 1))
+    with ThreadPoolExecutor(workers) as ex:
+        norm2 = sum(ex.map(fill, range(nblk)))  # summed in block order
+    scale = 1.0 / np.sqrt(norm2)
+
+    def rescale(b):
+        c[b * BLOCK:(b + 1) * BLOCK] *= scale
+
+    with ThreadPoolExecutor(workers) as ex:
+        list(ex.map(rescale, range(nblk)))
     return c
 
 

@@ -34,13 +34,23 @@ extern "C" {
 /* ---- engine options ---------------------------------------------------- */
 /* Arithmetic of the FFT passes.  The interaction table, its DCT-I and every
  * overlap sum are FP64 in both modes. */
-#define CLAT_PREC_FP64 0  /* complex128 passes, phases and kernel multiply */
-#define CLAT_PREC_FP32 1  /* complex64 passes, phases and kernel multiply  */
+#define CLAT_PREC_FP64 0  /* complex128 passes, phases and kernel multiply:
+                             every integral within 1e-10 of itself against
+                             the FP64 CPU restatement of the reference      */
+#define CLAT_PREC_FP32 1  /* complex64 passes, phases and kernel multiply:
+                             every integral within 1e-5 of max(|I|, 1e-3
+                             max|I|) against FP64                           */
 #define CLAT_PREC_FP32X 2 /* complex64 storage and arithmetic in every pass
                              but the last; the last (inverse x) pass computes
-                             in FP64 and writes the potential in FP64, which
-                             keeps every integral within 1e-5 of FP64 at
-                             complex64 traffic                              */
+                             in FP64 and writes the potential in FP64.  On the
+                             1M-atom, 8-state workload every integral is
+                             within 1e-5 of itself against FP64 (5.6e-6).  In
+                             general: within 1e-5 of itself for integrals of
+                             at least 1e-6 of the run's largest, within 1e-11
+                             of the largest below that -- rounding a pair
+                             density to complex64 at all moves integrals that
+                             cancel to ~1e-8 of the largest by more than 1e-5
+                             of themselves                                  */
 
 /* Schedule of the eight half-shift rounds (CoulombCalculator.cpp:105-163). */
 #define CLAT_SCHED_SHARED 0  /* rounds share their x and y passes: the eight

@@ -8,6 +8,7 @@ import pytest
 import clat
 import oracle as O
 import zincblende as zb
+from parity import TOL, check, rel_err
 
 pytestmark = pytest.mark.gpu
 
@@ -15,8 +16,8 @@ COMBOS = [(clat.PREC_FP64, clat.SCHED_SHARED), (clat.PREC_FP64, clat.SCHED_ROUND
           (clat.PREC_FP32, clat.SCHED_SHARED), (clat.PREC_FP32, clat.SCHED_ROUNDS),
           (clat.PREC_FP32X, clat.SCHED_SHARED), (clat.PREC_FP32X, clat.SCHED_ROUNDS)]
 IDS = ["fp64-shared", "fp64-rounds", "fp32-shared", "fp32-rounds", "fp32x-shared", "fp32x-rounds"]
-# relative tolerance per precision: FP64 path 1e-10 (vs oracle), complex64 paths 1e-5
-TOL = {clat.PREC_FP64: 1e-10, clat.PREC_FP32: 1e-5, clat.PREC_FP32X: 1e-5}
+# per-integral bars (tests/parity.py): FP64 1e-10 and FP32X 1e-5 relative to
+# each integral, no floor; pure FP32 1e-5 relative to max(|I|, 1e-3 max|I|)
 
 
 def _mock(G):
@@ -93,8 +94,7 @@ def test_column_kernels_match_oracle(prec, sched, dims):
 
 
 def _rel(a, b):
-    scale = np.max(np.abs(b))
-    return np.max(np.abs(a - b) / np.maximum(np.abs(b), 1e-3 * scale))
+    return float(np.max(rel_err(a, b)))
 
 
 def _example():
@@ -142,9 +142,7 @@ def test_zincblende_integrals(prec, sched):
     rho = O.products(np.stack(w["states"]))
     Gh = O.ghat(O.gtable(X, Y, Z, w["step"], 0.0, 10.89))
     want = O.compute_integrals([tuple(q) for q in quads], 4, rho, w["cells"], (X, Y, Z), Gh)
-    scale = np.max(np.abs(want))
-    err = np.abs(got - want) / np.maximum(np.abs(want), 1e-3 * scale)
-    assert np.max(err) < TOL[prec], np.max(err)
+    check(got, want, prec, quads)
 
 
 @pytest.mark.parametrize("prec,sched", [COMBOS[0], COMBOS[1], COMBOS[4]], ids=[IDS[0], IDS[1], IDS[4]])
@@ -173,8 +171,8 @@ def test_stale_values(prec, sched):
     changed = np.abs(want_stale - want) > 1e-6 * np.max(np.abs(want))
     assert list(np.nonzero(changed)[0]) == [3]
     assert want_stale[3] == np.conj(want_stale[4])
-    assert _rel(got_stale, want_stale) < TOL[prec], _rel(got_stale, want_stale)
-    assert _rel(got, want) < TOL[prec], _rel(got, want)
+    check(got_stale, want_stale, prec, quads)
+    check(got, want, prec, quads)
     assert got_stale[3] == np.conj(got_stale[4])
 
 
@@ -216,7 +214,7 @@ def test_atoms_sharing_cells(prec, sched):
     rho = O.products(st[:, eff, :])
     Gh = O.ghat(O.gtable(X, Y, Z, w["step"], 0.0, 10.89))
     want = O.compute_integrals([tuple(q) for q in quads], 4, rho, cells[eff], (X, Y, Z), Gh)
-    assert _rel(got, want) < TOL[prec], _rel(got, want)
+    check(got, want, prec, quads)
     # the same run with every atom counted differs: the collisions matter here
     naive = O.compute_integrals([tuple(q) for q in quads], 4, O.products(st), cells, (X, Y, Z), Gh)
     assert _rel(naive, want) > 1e-3
@@ -276,7 +274,7 @@ def test_empty_and_invalid_integral_lists():
     rho = O.products(np.stack(w["states"]))
     Gh = O.ghat(O.gtable(X, Y, Z, w["step"], 0.0, 10.89))
     want = O.compute_integrals([(1, 2, 2, 1), (2, 1, 1, 2)], 2, rho, w["cells"], (X, Y, Z), Gh)
-    assert _rel(got, want) < TOL[clat.PREC_FP32X]
+    check(got, want, clat.PREC_FP32X)
 
 
 @pytest.mark.parametrize("prec", [clat.PREC_FP64, clat.PREC_FP32, clat.PREC_FP32X], ids=["fp64", "fp32", "fp32x"])
@@ -371,17 +369,14 @@ def test_batched_prepares(prec, monkeypatch):
     rho = O.products(np.stack(w["states"]))
     Gh = O.ghat(O.gtable(X, Y, Z, w["step"], 0.0, 10.89))
     want = O.compute_integrals([tuple(q) for q in quads], 5, rho, w["cells"], (X, Y, Z), Gh)
-    scale = np.max(np.abs(want))
-    err = np.abs(got - want) / np.maximum(np.abs(want), 1e-3 * scale)
-    assert np.max(err) < TOL[prec], np.max(err)
+    check(got, want, prec, quads)
     # overlap sums on FP64 MFMA tiles (forced here; 15 slots x 15 products fill
     # one 64 x 64 tile partially): the same sums in another order
     monkeypatch.delenv("CLAT_PACK")
     monkeypatch.setenv("CLAT_OVL", "mfma")
     on_mfma = run()
     assert _rel(on_mfma, got) < 1e-12, _rel(on_mfma, got)
-    err = np.abs(on_mfma - want) / np.maximum(np.abs(want), 1e-3 * scale)
-    assert np.max(err) < TOL[prec], np.max(err)
+    check(on_mfma, want, prec, quads)
     # the default here is the 36 x 36 vector tile (three site groups per
     # block); the 48 x 48 tile sums the same terms in another order
     monkeypatch.setenv("CLAT_OVL", "valu48")
@@ -434,9 +429,7 @@ def test_thomas_fermi(prec):
     rho = O.products(np.stack(w["states"]))
     Gh = O.ghat(O.gtable(X, Y, Z, w["step"], 0.0, 10.89, tf_lattice=5.6533))
     want = O.compute_integrals([tuple(q) for q in quads], 2, rho, w["cells"], (X, Y, Z), Gh)
-    scale = np.max(np.abs(want))
-    err = np.abs(got - want) / np.maximum(np.abs(want), 1e-3 * scale)
-    assert np.max(err) < TOL[prec]
+    check(got, want, prec, quads)
 
 
 @pytest.mark.parametrize("prec,sched", COMBOS, ids=IDS)
@@ -458,7 +451,5 @@ def test_golden_direct_sums(prec, sched, golden_dir):
             eng.add_state(s)
         got = eng.compute_integrals(case["quads"])
         want = np.array([complex(a, b) for a, b in case["values"]])
-        scale = np.max(np.abs(want))
-        err = np.abs(got - want) / np.maximum(np.abs(want), 1e-3 * scale)
-        assert np.max(err) < TOL[prec], (case["name"], np.max(err))
+        check(got, want, prec, case["quads"])
         eng.close()
