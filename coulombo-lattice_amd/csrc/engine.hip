@@ -1098,6 +1098,46 @@ __global__ void k_atoms_from_sites(const double2* __restrict__ vs, const int32_t
     out[t] = vs[atom_site[t]];
 }
 
+// One plain prepare per launch with no packed second density: the state the
+// pass launches read outside a batched run.
+void reset_batch(clat_engine* e) {
+  e->cur_nb = 1;
+  for (int b = 0; b < LINE_NB; ++b) {
+    e->b_in2[b] = nullptr;
+    e->b_out2[b] = nullptr;
+  }
+}
+
+struct BatchScope {
+  clat_engine* e;
+  explicit BatchScope(clat_engine* x) : e(x) {}
+  ~BatchScope() { reset_batch(e); }
+  BatchScope(const BatchScope&) = delete;
+  BatchScope& operator=(const BatchScope&) = delete;
+};
+
+// V of state f's self density at every atom's cell (potentials.cpp:140-158):
+// the self density at the sites into rho, its potential at the sites into vs,
+// the per-atom values into va and from there to the host array out.  The
+// three device buffers hold n_sites, n_sites and n_atoms elements; the caller
+// owns them.  Synchronous.
+void potential_run(clat_engine* e, int f, double2* rho, double2* vs, double2* va, double* out) {
+  reset_batch(e);
+  if (e->n_sites > 0) {
+    hipLaunchKernelGGL(k_self_density, dim3(grid1d(e->n_sites)), dim3(256), 0, e->stream, e->coeffs, e->n_sites,
+                       e->n_orb, f, rho);
+    LAUNCH_CHECK();
+    prepare_sites(e, rho, vs);
+  }
+  if (e->n_atoms > 0) {
+    hipLaunchKernelGGL(k_atoms_from_sites, dim3(grid1d(e->n_atoms)), dim3(256), 0, e->stream, vs, e->d_atom_site,
+                       va, e->n_atoms);
+    LAUNCH_CHECK();
+    HIPX(hipMemcpyAsync(out, va, (size_t)e->n_atoms * sizeof(double2), hipMemcpyDeviceToHost, e->stream));
+  }
+  HIPX(hipStreamSynchronize(e->stream));
+}
+
 void free_atoms(clat_engine* e) {
   e->dfree(e->d_site_atom, (size_t)e->n_sites * 4);
   e->dfree(e->d_site_cell, (size_t)e->n_sites * 4);
@@ -1703,6 +1743,9 @@ int clat_compute_integrals(clat_engine* e, const int32_t* quads, int64_t n, int 
     const int nU = (int)units.size();
     for (int u = 0; u < nU && e->n_sites > 0;) {
       const int nb = std::min(e->nb_max, nU - u);
+      // the batch lives in engine state for the pass launches; the guard
+      // returns it to one plain prepare however this scope is left
+      BatchScope batch(e);
       e->cur_nb = nb;
       for (int b = 0; b < nb; ++b) {
         const int s0 = units[(size_t)(u + b)][0], s1 = units[(size_t)(u + b)][1];
@@ -1712,11 +1755,6 @@ int clat_compute_integrals(clat_engine* e, const int32_t* quads, int64_t n, int 
         e->b_out2[b] = s1 >= 0 ? e->Vat + (size_t)s1 * e->n_sites : nullptr;
       }
       prepare_sites(e, e->b_in[0], e->b_out[0]);
-      for (int b = 0; b < LINE_NB; ++b) {
-        e->b_in2[b] = nullptr;
-        e->b_out2[b] = nullptr;
-      }
-      e->cur_nb = 1;
       u += nb;
     }
     HIPX(hipEventRecord(e->stage_ev[3], e->stream));
@@ -1827,31 +1865,19 @@ int clat_compute_potential_at_atoms(clat_engine* e, int state, double* v) {
   if (!e->initialized) return set_err(CLAT_ESTATE, "calculator is not initialized");
   if (state < 1 || state > e->n_states) return set_err(CLAT_EINVAL, "state index out of range");
   return guarded([&] {
-    build_products(e);
-    const int f = state - 1;
-    const size_t ns = (size_t)std::max<int64_t>(1, e->n_sites), na = (size_t)std::max<int64_t>(1, e->n_atoms);
-    double2* vs = (double2*)e->dalloc(ns * sizeof(double2));
-    double2* va = (double2*)e->dalloc(na * sizeof(double2));
-    hipError_t ce = hipSuccess;
+    // self density, V at the sites and V at the atoms: 2 n_sites + n_atoms
+    // elements, freed at return
+    const size_t ns = (size_t)e->n_sites, na = (size_t)e->n_atoms;
+    const size_t bytes = std::max<size_t>(1, 2 * ns + na) * sizeof(double2);
+    double2* buf = (double2*)e->dalloc(bytes);
     try {
-      if (e->n_sites > 0) prepare_sites(e, e->rho + (size_t)(f * (f + 1) / 2 + f) * e->n_sites, vs);
-      if (e->n_atoms > 0) {
-        hipLaunchKernelGGL(k_atoms_from_sites, dim3(grid1d(e->n_atoms)), dim3(256), 0, e->stream, vs,
-                           e->d_atom_site, va, e->n_atoms);
-        LAUNCH_CHECK();
-        ce = hipMemcpyAsync(v, va, (size_t)e->n_atoms * sizeof(double2), hipMemcpyDeviceToHost, e->stream);
-      }
+      potential_run(e, state - 1, buf, buf + ns, buf + 2 * ns, v);
     } catch (...) {
       (void)hipStreamSynchronize(e->stream);
-      e->dfree(vs, ns * sizeof(double2));
-      e->dfree(va, na * sizeof(double2));
+      e->dfree(buf, bytes);
       throw;
     }
-    hipError_t se = hipStreamSynchronize(e->stream);
-    e->dfree(vs, ns * sizeof(double2));
-    e->dfree(va, na * sizeof(double2));
-    HIPX(ce);
-    HIPX(se);
+    e->dfree(buf, bytes);
     e->kcollect();
   });
 }

@@ -559,6 +559,24 @@ __global__ __launch_bounds__(NT) void k_products(const double2* __restrict__ coe
   }
 }
 
+// The self density of one state at the sites, sum_o |c_f(site, o)|^2: the
+// product a potential of that state needs (FunctionCollection::
+// createSelfProducts, FunctionCollection.cpp:108-120, + Product::map,
+// Product.cpp:16-30), without forming the
+// other n(n+1)/2 - 1 pair densities.  Real: the imaginary part is zero.
+__global__ void k_self_density(const double2* __restrict__ coeffs, int64_t n_sites, int n_orb, int f,
+                               double2* __restrict__ rho) {
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n_sites; t += (int64_t)gridDim.x * blockDim.x) {
+    const double2* c = coeffs + ((size_t)f * n_sites + t) * n_orb;
+    double re = 0.0;
+    for (int o = 0; o < n_orb; ++o) {
+      const double2 x = c[o];
+      re += x.x * x.x + x.y * x.y;
+    }
+    rho[t] = make_double2(re, 0.0);
+  }
+}
+
 // out[(k*n_orb + o)] = in[(atom[k]*n_orb + o)]: one state's coefficients in site order
 __global__ void k_coeffs_to_sites(const double2* __restrict__ in, const int32_t* __restrict__ atom,
                                   int64_t n_sites, int n_orb, double2* __restrict__ out) {
