@@ -38,6 +38,7 @@ SYMBOLS = [
     "clat_set_interaction_simple", "clat_set_interaction_thomas_fermi", "clat_set_interaction_table",
     "clat_set_input", "clat_prepare", "clat_calculate", "clat_get_potential", "clat_set_atoms",
     "clat_add_state", "clat_clear_states", "clat_compute_integrals", "clat_compute_potential_at_atoms",
+    "clat_compute_potential_at_atoms_noalloc",
     "clat_last_stage_ms", "clat_set_kernel_timing", "clat_set_kernel_timing_kinds", "clat_kernel_kind_count",
     "clat_kernel_kind_name",
     "clat_kernel_stats", "clat_reset_kernel_stats", "clat_engine_info",
@@ -84,6 +85,7 @@ def lib() -> ctypes.CDLL:
     L.clat_clear_states.argtypes = [vp]
     L.clat_compute_integrals.argtypes = [vp, vp, _i64, ctypes.c_int, vp]
     L.clat_compute_potential_at_atoms.argtypes = [vp, ctypes.c_int, _dp]
+    L.clat_compute_potential_at_atoms_noalloc.argtypes = [vp, ctypes.c_int, _dp]
     L.clat_last_stage_ms.argtypes = [vp, _dp]
     L.clat_set_kernel_timing.argtypes = [vp, ctypes.c_int]
     L.clat_set_kernel_timing_kinds.argtypes = [vp, ctypes.c_uint32]
@@ -314,6 +316,16 @@ class Engine:
     def potential_at_atoms(self, state: int) -> np.ndarray:
         out = np.zeros(self.n_atoms, np.complex128)
         _check(lib().clat_compute_potential_at_atoms(self.handle, state, out.view(np.float64).reshape(-1)))
+        return out
+
+    def potential_at_atoms_noalloc(self, state: int, out: np.ndarray | None = None) -> np.ndarray:
+        """The same values through the allocation-free entry point (scratch
+        in the library's static device array); out, if given, is reused."""
+        if out is None:
+            out = np.zeros(self.n_atoms, np.complex128)
+        if out.dtype != np.complex128 or out.shape != (self.n_atoms,) or not out.flags.c_contiguous:
+            raise ValueError("out must be a contiguous complex128 array of n_atoms values")
+        _check(lib().clat_compute_potential_at_atoms_noalloc(self.handle, state, out.view(np.float64).reshape(-1)))
         return out
 
     def stage_ms(self) -> np.ndarray:

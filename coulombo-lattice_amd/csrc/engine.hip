@@ -8,6 +8,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <array>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -1138,6 +1139,42 @@ void potential_run(clat_engine* e, int f, double2* rho, double2* vs, double2* va
   HIPX(hipStreamSynchronize(e->stream));
 }
 
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// scratch of the allocation-free potential entry point: one static device
+// array, held by every GPU the library runs on until the process ends.  Host
+// code never uses its name as an address (that is a host-side stand-in); it
+// uses the per-device address that hipGetSymbolAddress returns.
+// ---------------------------------------------------------------------------
+constexpr size_t NOALLOC_ELEMS = (size_t)1 << 22;  // double2 elements: 64 MiB
+static __device__ __align__(256) double2 scratch[NOALLOC_ELEMS];
+
+namespace {
+
+constexpr int MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+double2* g_scratch_ptr[MAX_DEVICES] = {};
+
+// The address of `scratch` on device dev, looked up on the first call for
+// that device with the device current (each GPU holds its own copy).
+double2* scratch_of(int dev) {
+  if (dev < 0 || dev >= MAX_DEVICES) throw host::Error(CLAT_EDEVICE, "device index out of range");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    int cur = -1;
+    HIPX(hipGetDevice(&cur));
+    if (cur != dev) HIPX(hipSetDevice(dev));
+    void* p = nullptr;
+    const hipError_t rc = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (cur != dev) (void)hipSetDevice(cur);
+    HIPX(rc);
+    if (!p) throw host::Error(CLAT_EDEVICE, "scratch symbol has no device address");
+    g_scratch_ptr[dev] = (double2*)p;
+  }
+  return g_scratch_ptr[dev];
+}
+
 void free_atoms(clat_engine* e) {
   e->dfree(e->d_site_atom, (size_t)e->n_sites * 4);
   e->dfree(e->d_site_cell, (size_t)e->n_sites * 4);
@@ -1879,6 +1916,34 @@ int clat_compute_potential_at_atoms(clat_engine* e, int state, double* v) {
     }
     e->dfree(buf, bytes);
     e->kcollect();
+  });
+}
+
+int clat_compute_potential_at_atoms_noalloc(clat_engine* e, int state, double* v) {
+  if (!e || !v) return set_err(CLAT_EINVAL, "invalid arguments");
+  if (!e->initialized) return set_err(CLAT_ESTATE, "calculator is not initialized");
+  if (state < 1 || state > e->n_states) return set_err(CLAT_EINVAL, "state index out of range");
+  if (!e->fast)
+    return set_err(CLAT_EUNSUP, "the allocation-free potential needs the two-level pass kernels; this grid "
+                                "runs on the general kernels, whose dense grids are allocated on first use");
+  const double need = 2.0 * (double)e->n_sites + (double)e->n_atoms;
+  if (need > (double)NOALLOC_ELEMS)
+    return set_err(CLAT_EUNSUP, "the allocation-free potential needs 2 x %lld sites + %lld atoms = %.0f "
+                                "scratch elements; its maximum is %zu (64 MiB)",
+                   (long long)e->n_sites, (long long)e->n_atoms, need, NOALLOC_ELEMS);
+  return guarded([&] {
+    double2* s = scratch_of(e->device);
+    const size_t ns = (size_t)e->n_sites;
+    // no kernel timing here: its events and records would allocate
+    const uint32_t kmask = e->kmask;
+    e->kmask = 0;
+    try {
+      potential_run(e, state - 1, s, s + ns, s + 2 * ns, v);
+    } catch (...) {
+      e->kmask = kmask;
+      throw;
+    }
+    e->kmask = kmask;
   });
 }
 

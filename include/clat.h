@@ -183,6 +183,21 @@ int clat_compute_integrals(clat_engine* e, const int32_t* quads, int64_t n,
  * FunctionCollection.cpp:122-131).  v: n_atoms complex values. */
 int clat_compute_potential_at_atoms(clat_engine* e, int state, double* v);
 
+/* The same potential without any device allocation, for callers that cannot
+ * allocate (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  No counterpart in the reference.  The self density, the
+ * potential at the sites and the per-atom values live in one static device
+ * array of 4,194,304 complex elements (64 MiB) that every GPU running the
+ * library holds until the process ends: 2 * (occupied cells) + n_atoms must
+ * not exceed that, else CLAT_EUNSUP with the sizes in the message.  Needs an
+ * engine on the two-level pass kernels (every axis of at most 400 points
+ * splitting into two factors of at most 20; else CLAT_EUNSUP).  Kernel timing
+ * is not recorded during the call.  Calls on one GPU share that array: calls
+ * that overlap in time on one GPU (engines driven from several threads) race
+ * and must be serialised by the caller.  Same values as
+ * clat_compute_potential_at_atoms (same kernels). */
+int clat_compute_potential_at_atoms_noalloc(clat_engine* e, int state, double* v);
+
 /* ---- measurement ------------------------------------------------------- */
 
 /* Stage times in milliseconds from HIP events on the engine stream:
