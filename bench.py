@@ -10,7 +10,19 @@ HBM: interaction table + DCT-I, pair densities, the 36 prepares, every
 overlap, the results back on the host.  value = integrals / second of the
 complex64-storage / FP64-arithmetic path (CLAT_PREC_FP32X).  The FP64 path,
 the plain FP64 path (eight independent rounds) and pure FP32 run the same
-inputs for the accuracy and speed comparison.
+inputs for the accuracy and speed comparison; every FP32X integral's error
+against the FP64 path is summarised by magnitude (error_distribution) and,
+with --error-report FILE, written one line per integral.
+
+Beside the headline, on the same JSON line (omitted with --headline-only):
+  batched_form  32^3 cells = 262,144 atoms on a 128^3 grid, 32 states (16
+                holes, 16 electrons), the full ijkl set (1,048,576 integrals);
+  multi_system  six boxes of 16^3, 24^3, 30^3, 40^3, 48^3 and 100^3 cells
+                (grids 64^3 ... 400^3, up to 8M atoms), 8 states and 20
+                orbitals each, eeee,hhhh,ehhe,eheh; one engine per box, all
+                boxes' steps run concurrently (one host thread and stream each).
+Each carries ms/step, integrals/s, its dominant kernel's roofline and the
+FP32X-vs-FP64 error summary.
 
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--cells NX NY NZ]
 """
@@ -24,10 +36,48 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "coulombo-lattice_amd"))
 
 PEAK_HBM_GBS = 8000.0
+PEAK_FP64_TFLOPS = 78.6
+FFT_KINDS = ("xfwd", "yfwd", "zconv", "yinv", "xinv")
 
 
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
+
+
+def reference_deps():
+    """Whether this host could build and run the reference coulombo
+    (Makefile:1-3: FFTW3 with its MPI and OpenMP parts, armadillo, an MPI
+    compiler and launcher), probed by library and program name only."""
+    import ctypes.util
+    import shutil
+    libs = {name: ctypes.util.find_library(name) for name in
+            ("fftw3", "fftw3_mpi", "fftw3_omp", "armadillo")}
+    progs = {name: shutil.which(name) for name in ("mpirun", "mpic++")}
+    hdr = [h for h in ("fftw3.h", "fftw3-mpi.h", "armadillo")
+           if any(os.path.exists(os.path.join(d, h)) for d in ("/usr/include", "/usr/local/include"))]
+    ok = all(libs.values()) and all(progs.values()) and len(hdr) == 3
+    return {"buildable": ok, "libraries": libs, "programs": progs, "headers_found": hdr}
+
+
+def host_cores():
+    n_logical = os.cpu_count() or 1
+    try:
+        n_avail = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n_avail = n_logical
+    phys = set()
+    try:
+        with open("/proc/cpuinfo") as f:
+            pid = core = None
+            for line in f:
+                if line.startswith("physical id"):
+                    pid = line.split(":")[1].strip()
+                elif line.startswith("core id"):
+                    core = line.split(":")[1].strip()
+                    phys.add((pid, core))
+    except OSError:
+        pass
+    return {"logical": n_logical, "available_to_process": n_avail, "physical": len(phys) or None}
 
 
 def cpu_baseline(w, quads, budget_s=12.0):
@@ -69,10 +119,16 @@ def cpu_baseline(w, quads, budget_s=12.0):
             break
     total = t_init + t_rho + len(lefts) * t_prep / n_prep + len(steps) * t_ovl / max(1, n_ovl)
     sampled = t_init + t_rho + t_prep + t_ovl
+    deps = reference_deps()
     return {"value": len(quads) / total, "unit": "integrals/s", "cores": O.num_threads(), "kind": "port",
-            "sample": "FP64 oracle on the same workload: interaction table + DCT-I, all pair densities, "
-                      "%d of %d prepares and their %d overlaps (%.1f s of CPU work), scaled to the %d-integral "
-                      "run" % (n_prep, len(lefts), n_ovl, sampled, len(quads))}
+            "sample": "FP64 CPU restatement of the reference path (oracle/oracle.c: the same eight-round "
+                      "method with its own recursive mixed-radix FFT, OpenMP; not the reference binary, "
+                      "whose FFTW3/armadillo/MPI are %s on this host, so it understates an FFTW run) on "
+                      "the same workload: interaction table + DCT-I, all pair densities, %d of %d prepares "
+                      "and their %d overlaps (%.1f s of CPU work), scaled to the %d-integral run"
+                      % ("present" if deps["buildable"] else "absent", n_prep, len(lefts), n_ovl, sampled,
+                         len(quads)),
+            "host_cores": host_cores(), "reference_deps": deps}
 
 
 def traffic_of(kind, default_workload):
@@ -91,24 +147,215 @@ def traffic_of(kind, default_workload):
         return None
 
 
+def error_summary(np, got, ref, quads, small=1e-6):
+    """Per-integral relative error of got against ref (no floor), by decade
+    of |I| / max|I|, with the largest errors named."""
+    e = np.abs(got - ref) / np.abs(ref)
+    scale = float(np.max(np.abs(ref)))
+    mag = np.abs(ref) / scale
+    bins = []
+    for lo in range(0, -14, -1):
+        sel = (mag <= 10.0 ** lo) & (mag > 10.0 ** (lo - 1))
+        if sel.any():
+            bins.append({"mag_range": "(1e%d, 1e%d]" % (lo - 1, lo), "integrals": int(sel.sum()),
+                         "max_rel_err": float(np.max(e[sel])), "median_rel_err": float(np.median(e[sel]))})
+    big = mag >= small
+    worst = []
+    for i in np.argsort(-e)[:5]:
+        worst.append({"quad": [int(v) for v in quads[i]], "mag": float(mag[i]), "rel_err": float(e[i]),
+                      "abs_err_over_max": float(abs(got[i] - ref[i]) / scale)})
+    return {"integrals": int(len(e)), "max_rel_err": float(np.max(e)), "median_rel_err": float(np.median(e)),
+            "n_over_1e-5": int(np.sum(e > 1e-5)),
+            "max_rel_err_at_least_1e-6_of_max": float(np.max(e[big])) if big.any() else 0.0,
+            "max_abs_err_over_max_below_1e-6_of_max":
+                float(np.max(np.abs(got - ref)[~big]) / scale) if (~big).any() else 0.0,
+            "by_magnitude": bins, "worst": worst}
+
+
+def make_engine(clat, w, prec, sched, n_orb):
+    X, Y, Z = w["dims"]
+    eng = clat.Engine((X, Y, Z), prec, sched)
+    eng.set_atoms(w["cells"], n_orb)
+    for s in w["states"]:
+        eng.add_state(s)
+    return eng
+
+
+def timed(eng, step, steps, warmup, dominant=False, breakdown=False):
+    """warmup untimed steps, then `steps` timed ones.  With dominant=True one
+    untimed step with every kernel timed first names the dominant kernel and
+    the timed steps carry HIP events around its launches only (events around
+    every launch would cost ~0.25 ms per step); breakdown=True adds two more
+    untimed steps with every kernel timed (per-kernel ms per step)."""
+    for _ in range(warmup):
+        step(eng)
+    dom = None
+    if dominant:
+        eng.set_kernel_timing(True)
+        eng.reset_kernel_stats()
+        step(eng)
+        pre = eng.kernel_stats()
+        dom = max(pre, key=lambda k: pre[k][1])
+        eng.set_kernel_timing_kinds([dom])
+    eng.reset_kernel_stats()
+    vals = None
+    t_all = time.perf_counter()
+    for _ in range(steps):
+        vals = step(eng)
+    t_all = time.perf_counter() - t_all
+    stats = eng.kernel_stats()
+    bd = None
+    if breakdown:
+        eng.set_kernel_timing(True)
+        eng.reset_kernel_stats()
+        for _ in range(2):
+            step(eng)
+        bd = {k: (v[0] / 2, v[1] / 2, v[2] / 2) for k, v in eng.kernel_stats().items()}
+    eng.set_kernel_timing(False)
+    return dict(vals=vals, total=t_all, steps=steps, dom=dom, stats=stats, breakdown=bd,
+                stage=eng.stage_ms().tolist())
+
+
+def roofline_of(r, flops_overlap=None, traffic=None):
+    """Roofline of the dominant kernel from the events around its launches:
+    HBM bytes for the passes, FP64 MFMA flops when the overlaps dominate."""
+    dom = r["dom"]
+    n_l, t_ms, byts = r["stats"][dom]
+    avg_ms = t_ms / max(1, n_l)
+    if dom == "overlap" and flops_overlap:
+        ach = flops_overlap / (avg_ms * 1e-3) / 1e12
+        return {"bound": "mfma", "kernel": dom, "achieved": ach, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+                "frac": ach / PEAK_FP64_TFLOPS, "traffic": traffic, "avg_launch_ms": avg_ms,
+                "algorithmic_flops_per_launch": flops_overlap}
+    ach = (byts / max(1, n_l)) / (avg_ms * 1e-3) / 1e9
+    return {"bound": "hbm", "kernel": dom, "achieved": ach, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+            "frac": ach / PEAK_HBM_GBS, "traffic": traffic, "avg_launch_ms": avg_ms,
+            "algorithmic_bytes_per_launch": byts / max(1, n_l)}
+
+
+def batched_form(args, clat, zb, np):
+    """32^3 cells, 16 holes + 16 electrons, 20 orbitals, ijkl: the batched form."""
+    t0 = time.time()
+    w = zb.workload(32, 32, 32, 16, 16, config=3, sigma_frac=0.15)
+    quads = clat.pattern_enumerate("ijkl", 32, 16)
+    X, Y, Z = w["dims"]
+    log("batched form: %d atoms, grid %dx%dx%d, 32 states, %d integrals (%.1f s)"
+        % (len(w["cells"]), X, Y, Z, len(quads), time.time() - t0))
+    inter = clat.InteractionSimple(w["step"], 0.0, 10.89)
+
+    def step(eng):
+        eng.initialize(inter)
+        return eng.compute_integrals(quads)
+
+    steps = max(1, args.steps // 2)
+    eng = make_engine(clat, w, clat.PREC_FP32X, clat.SCHED_SHARED, 20)
+    r = timed(eng, step, steps, max(1, args.warmup // 2), dominant=True)
+    eng.close()
+    eng = make_engine(clat, w, clat.PREC_FP64, clat.SCHED_SHARED, 20)
+    r64 = timed(eng, step, 1, 1)
+    eng.close()
+    n_left = len(np.unique(clat.plan(quads, 32)[:, 1]))
+    P = 32 * 33 // 2
+    ms = 1e3 * r["total"] / steps
+    log("batched form: FP32X %.2f ms per step, FP64 %.2f" % (ms, 1e3 * r64["total"]))
+    out = {"workload": "zincblende 32x32x32 cells = %d atoms, grid %dx%dx%d, 32 states, ijkl"
+                       % (len(w["cells"]), X, Y, Z),
+           "integrals": int(len(quads)), "steps": steps, "ms_per_step": ms,
+           "integrals_per_s": len(quads) / (r["total"] / steps),
+           "fp64_ms_per_step": 1e3 * r64["total"],
+           "stage_ms": dict(zip(("table", "pair_densities", "prepares", "overlaps", "call"), r["stage"])),
+           "roofline": roofline_of(r, 8.0 * n_left * P * len(w["cells"])),
+           "fp32x_vs_fp64": error_summary(np, r["vals"], r64["vals"], quads)}
+    return out
+
+
+def multi_system_batch(args, clat, zb, np, sizes=(16, 24, 30, 40, 48, 100)):
+    """Six boxes, 4 holes + 4 electrons, 20 orbitals, eeee,hhhh,ehhe,eheh; one
+    engine per box, every box's step run concurrently (run_concurrent), and
+    one after another for comparison."""
+    quads = clat.pattern_enumerate("eeee,hhhh,ehhe,eheh", 8, 4)
+    systems = []
+    for n in sizes:
+        t0 = time.time()
+        w = zb.workload(n, n, n, 4, 4, config=4, sigma_frac=0.15, n_orb=20)
+        systems.append(w)
+        log("multi-system box %d^3 cells: grid %dx%dx%d, %d atoms (%.1f s)"
+            % (n, *w["dims"], len(w["cells"]), time.time() - t0))
+
+    def job_of(eng, w):
+        inter = clat.InteractionSimple(w["step"], 0.0, 10.89)
+
+        def job():
+            eng.initialize(inter)
+            return eng.compute_integrals(quads)
+        return job
+
+    engs = [make_engine(clat, w, clat.PREC_FP32X, clat.SCHED_SHARED, 20) for w in systems]
+    jobs = [job_of(e, w) for e, w in zip(engs, systems)]
+    steps = max(1, args.steps // 2)
+    for _ in range(max(1, args.warmup // 2)):
+        clat.run_concurrent(jobs)
+    # the largest box names the dominant kernel (it takes most of the step)
+    big = engs[-1]
+    big.set_kernel_timing(True)
+    big.reset_kernel_stats()
+    jobs[-1]()
+    pre = big.kernel_stats()
+    dom = max(pre, key=lambda k: pre[k][1])
+    big.set_kernel_timing_kinds([dom])
+    big.reset_kernel_stats()
+    t = time.perf_counter()
+    for _ in range(steps):
+        con = clat.run_concurrent(jobs)
+    t_con = (time.perf_counter() - t) / steps
+    stats = big.kernel_stats()
+    big.set_kernel_timing(False)
+    t = time.perf_counter()
+    seq = [j() for j in jobs]
+    t_seq = time.perf_counter() - t
+    per_box = []
+    for e, w, j in zip(engs, systems, jobs):
+        t = time.perf_counter()
+        j()
+        per_box.append(1e3 * (time.perf_counter() - t))
+    for e in engs:
+        e.close()
+    same = all(np.array_equal(a, b) for a, b in zip(seq, con))
+    errs = []
+    for w, v in zip(systems, con):
+        e64 = make_engine(clat, w, clat.PREC_FP64, clat.SCHED_SHARED, 20)
+        ref = job_of(e64, w)()
+        e64.close()
+        s = error_summary(np, v, ref, quads)
+        errs.append({"grid": "%dx%dx%d" % tuple(w["dims"]), "atoms": int(len(w["cells"])),
+                     "max_rel_err": s["max_rel_err"], "n_over_1e-5": s["n_over_1e-5"],
+                     "max_rel_err_at_least_1e-6_of_max": s["max_rel_err_at_least_1e-6_of_max"],
+                     "max_abs_err_over_max_below_1e-6_of_max": s["max_abs_err_over_max_below_1e-6_of_max"],
+                     "worst": s["worst"][:2]})
+    n_int = len(quads) * len(systems)
+    log("multi-system: concurrent %.2f ms per step, one after another %.2f" % (1e3 * t_con, 1e3 * t_seq))
+    r = {"dom": dom, "stats": stats}
+    return {"workload": "zincblende boxes of %s cells, 8 states and 20 orbitals each, eeee,hhhh,ehhe,eheh"
+                        % "/".join("%d^3" % n for n in sizes),
+            "integrals": n_int, "steps": steps, "ms_per_step": 1e3 * t_con, "integrals_per_s": n_int / t_con,
+            "sequential_ms_per_step": 1e3 * t_seq, "box_ms": per_box,
+            "concurrent_equals_sequential": bool(same),
+            "roofline_largest_box": roofline_of(r), "fp32x_vs_fp64": errs}
+
+
 def multi_system(args, clat, zb, np):
     """Several zincblende systems of uneven size, one engine each (FP32X): the
     step is every system's interaction build and integrals, run one after
-    another and then concurrently (one host thread and one stream per engine).
-    Four orbitals per atom keep the host-side generation of the 8M-atom
-    coefficients short; only the pair-density kernel reads them."""
+    another and then concurrently (one host thread and one stream per engine)."""
     sizes = [int(v) for v in args.systems.split(",")]
     nh, ne = args.states
-    n_orb = 4
+    n_orb = 20
     systems = []
     for k, n in enumerate(sizes):
         t0 = time.time()
         w = zb.workload(n, n, n, nh, ne, config=4, sigma_frac=0.15, n_orb=n_orb)
         X, Y, Z = w["dims"]
-        eng = clat.Engine((X, Y, Z), clat.PREC_FP32X, clat.SCHED_SHARED)
-        eng.set_atoms(w["cells"], n_orb)
-        for st in w["states"]:
-            eng.add_state(st)
+        eng = make_engine(clat, w, clat.PREC_FP32X, clat.SCHED_SHARED, n_orb)
         quads = clat.pattern_enumerate(args.integrals, nh + ne, nh)
         inter = clat.InteractionSimple(w["step"], 0.0, 10.89)
         systems.append((eng, inter, quads, (X, Y, Z), len(w["cells"])))
@@ -128,12 +375,10 @@ def multi_system(args, clat, zb, np):
     for _ in range(args.steps):
         seq = [job(sy) for sy in systems]
     t_seq = (time.perf_counter() - t) / args.steps
-    log("one after another: %.2f ms per step" % (1e3 * t_seq))
     t = time.perf_counter()
     for _ in range(args.steps):
         con = clat.run_concurrent([(lambda sy=sy: job(sy)) for sy in systems])
     t_con = (time.perf_counter() - t) / args.steps
-    log("concurrent: %.2f ms per step" % (1e3 * t_con))
     same = all(np.array_equal(a, b) for a, b in zip(seq, con))
     n_int = sum(len(sy[2]) for sy in systems)
     out = {
@@ -163,12 +408,16 @@ def main():
     ap.add_argument("--states", type=int, nargs=2, default=[4, 4])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--headline-only", action="store_true",
+                    help="skip the batched-form and multi-system entries of the JSON line")
+    ap.add_argument("--error-report", default="", metavar="FILE",
+                    help="write every headline integral with its FP32X and FP64 values and relative error")
     ap.add_argument("--schedule", choices=["shared", "rounds"], default="shared")
     ap.add_argument("--integrals", default="eeee,hhhh,ehhe,eheh",
                     help="integral pattern list (ijkl with --cells 32 32 32 --states 16 16: the batched form)")
     ap.add_argument("--systems", default="",
-                    help="comma-separated cubic box sizes in cells (e.g. 16,24,30,40,48,100): the multi-system "
-                         "workload, one engine per system, timed one after another and concurrently")
+                    help="comma-separated cubic box sizes in cells (e.g. 16,24,30,40,48,100): only the "
+                         "multi-system workload, one engine per system, its own JSON line")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a plain device-to-device copy of MIB MiB (counter calibration)")
     args = ap.parse_args()
@@ -204,14 +453,6 @@ def main():
     log("workload: %d atoms, grid %dx%dx%d, %d states, %d integrals (%.1f s)"
         % (len(w["cells"]), X, Y, Z, nh + ne, len(quads), time.time() - t0))
     sched = clat.SCHED_SHARED if args.schedule == "shared" else clat.SCHED_ROUNDS
-
-    def make(prec):
-        eng = clat.Engine((X, Y, Z), prec, sched)
-        eng.set_atoms(w["cells"], 20)
-        for s in w["states"]:
-            eng.add_state(s)
-        return eng
-
     inter = clat.InteractionSimple(w["step"], 0.0, 10.89)
 
     def step(eng):
@@ -227,48 +468,12 @@ def main():
         runs = runs[:1]
     res = {}
     for name, prec, sch in runs:
-        eng = clat.Engine((X, Y, Z), prec, sch)
-        eng.set_atoms(w["cells"], 20)
-        for s in w["states"]:
-            eng.add_state(s)
-        for _ in range(args.warmup):
-            step(eng)
+        eng = make_engine(clat, w, prec, sch, 20)
         head_run = name == "fp32x"
-        dom = None
-        if head_run:
-            # one untimed step with every kernel timed names the dominant
-            # kernel; the timed steps then carry events around its launches
-            # only (events around every launch cost ~0.25 ms per step)
-            eng.set_kernel_timing(True)
-            eng.reset_kernel_stats()
-            step(eng)
-            pre = eng.kernel_stats()
-            dom = max(pre, key=lambda k: pre[k][1])
-            eng.set_kernel_timing_kinds([dom])
-        eng.reset_kernel_stats()
         steps = args.steps if name in ("fp32x", "fp64") else max(1, args.steps // 3)
-        times = []
-        vals = None
-        t_all = time.perf_counter()
-        for _ in range(steps):
-            t = time.perf_counter()
-            vals = step(eng)
-            times.append(time.perf_counter() - t)
-        t_all = time.perf_counter() - t_all
-        stats = eng.kernel_stats()
-        breakdown = None
-        if head_run:
-            # per-kernel breakdown from two more (untimed) steps
-            eng.set_kernel_timing(True)
-            eng.reset_kernel_stats()
-            for _ in range(2):
-                step(eng)
-            breakdown = {k: (v[0] / 2, v[1] / 2, v[2] / 2) for k, v in eng.kernel_stats().items()}
-            eng.set_kernel_timing(False)
-        res[name] = dict(times=times, vals=vals, stats=stats, dom=dom, breakdown=breakdown,
-                         stage=eng.stage_ms().tolist(), total=t_all, steps=steps)
+        res[name] = timed(eng, step, steps, args.warmup, dominant=head_run, breakdown=head_run)
         log("%s: mean step %.2f ms  stages(ms) table %.2f rho %.2f prepares %.2f overlaps %.2f"
-            % (name, 1e3 * t_all / steps, *res[name]["stage"][:4]))
+            % (name, 1e3 * res[name]["total"] / steps, *res[name]["stage"][:4]))
         eng.close()
 
     head = res["fp32x"]
@@ -276,13 +481,7 @@ def main():
                         and args.integrals == "eeee,hhhh,ehhe,eheh" and args.schedule == "shared")
     ms = 1e3 * head["total"] / args.steps
     value = len(quads) / (head["total"] / args.steps)
-    # roofline of the dominant kernel, from HIP events around each of its
-    # launches in the timed steps
-    stats = head["stats"]
-    dom = head["dom"]
-    n_l, t_ms, byts = stats[dom]
-    avg_ms = t_ms / max(1, n_l)
-    achieved = (byts / max(1, n_l)) / (avg_ms * 1e-3) / 1e9
+    roof = roofline_of(head, traffic=traffic_of(head["dom"], default_workload))
     out = {
         "metric": "integrals/s, 1M-atom grid, FP32+FP64-acc vs FP64 at 1e-5 rel; FFT HBM GB/s",
         "value": value, "unit": "integrals/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
@@ -292,23 +491,19 @@ def main():
         "config": {"workload": "zincblende %dx%dx%d cells = %d atoms, grid %dx%dx%d, %d states, %s"
                                % (nx, ny, nz, len(w["cells"]), X, Y, Z, nh + ne, args.integrals),
                    "integrals": len(quads), "schedule": args.schedule},
-        "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                     "frac": achieved / PEAK_HBM_GBS, "traffic": traffic_of(dom, default_workload),
-                     "avg_launch_ms": avg_ms,
-                     "algorithmic_bytes_per_launch": byts / max(1, n_l)},
+        "roofline": roof,
         "kernel_ms_per_step": {k: v[1] for k, v in head["breakdown"].items() if v[0]},
         "fft_pass_gbs": {k: (v[2] / max(1e-12, v[1] * 1e-3)) / 1e9 for k, v in head["breakdown"].items()
-                         if v[0] and k in ("xfwd", "yfwd", "zconv", "yinv", "xinv")},
+                         if v[0] and k in FFT_KINDS},
     }
     # FFT HBM GB/s: measured HBM bytes per launch of each pass (the committed
     # FETCH_SIZE + WRITE_SIZE summary of this workload) times launches per
     # step, over the five passes' event-timed duration per step
-    fft_kinds = ("xfwd", "yfwd", "zconv", "yinv", "xinv")
-    fft_tr = {k: traffic_of(k, default_workload) for k in fft_kinds}
-    if all(fft_tr[k] for k in fft_kinds) and all(head["breakdown"][k][0] for k in fft_kinds):
+    fft_tr = {k: traffic_of(k, default_workload) for k in FFT_KINDS}
+    if all(fft_tr[k] for k in FFT_KINDS) and all(head["breakdown"][k][0] for k in FFT_KINDS):
         b = head["breakdown"]
-        hbm = sum(fft_tr[k] * b[k][0] for k in fft_kinds)
-        t_s = sum(b[k][1] for k in fft_kinds) * 1e-3
+        hbm = sum(fft_tr[k] * b[k][0] for k in FFT_KINDS)
+        t_s = sum(b[k][1] for k in FFT_KINDS) * 1e-3
         out["fft_hbm_gbs"] = {"value": hbm / t_s / 1e9, "peak": PEAK_HBM_GBS,
                               "frac": hbm / t_s / 1e9 / PEAK_HBM_GBS,
                               "bytes_per_step": hbm, "pass_ms_per_step": t_s * 1e3}
@@ -320,14 +515,30 @@ def main():
                 out["max_rel_err_%s_vs_fp64" % name] = float(np.max(rel))
                 out["%s_ms_per_step" % name] = 1e3 * res[name]["total"] / res[name]["steps"]
         out["fp64_ms_per_step"] = 1e3 * res["fp64"]["total"] / res["fp64"]["steps"]
+        out["fp64_integrals_per_s"] = len(quads) / (res["fp64"]["total"] / res["fp64"]["steps"])
         out["speedup_vs_fp64_shared"] = out["fp64_ms_per_step"] / ms
         if "fp64_rounds" in res:
             out["speedup_vs_plain_fp64_rounds"] = out["fp64_rounds_ms_per_step"] / ms
+        out["error_distribution"] = error_summary(np, head["vals"], v64, quads)
+        if args.error_report:
+            with open(args.error_report, "w") as f:
+                f.write("# i1 i2 i3 i4  fp32x_re fp32x_im  fp64_re fp64_im  rel_err  |I|/max\n")
+                scale = np.max(np.abs(v64))
+                for q, a, b in zip(quads, head["vals"], v64):
+                    f.write("%2d %2d %2d %2d  %.17g %.17g  %.17g %.17g  %.3e  %.3e\n"
+                            % (*q, a.real, a.imag, b.real, b.imag, abs(a - b) / abs(b), abs(b) / scale))
     if not args.no_cpu_baseline:
         try:
             out["cpu_baseline"] = cpu_baseline(w, quads)
         except Exception as ex:  # reported, never fatal to the GPU number
             out["cpu_baseline"] = {"value": None, "error": str(ex)}
+    del w
+    if not args.headline_only:
+        for key, fn in (("batched_form", batched_form), ("multi_system", multi_system_batch)):
+            try:
+                out[key] = fn(args, clat, zb, np)
+            except Exception as ex:  # reported, never fatal to the headline
+                out[key] = {"error": "%s: %s" % (type(ex).__name__, ex)}
     print(json.dumps(out), flush=True)
 
 
