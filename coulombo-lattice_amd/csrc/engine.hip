@@ -275,7 +275,8 @@ struct clat_engine {
   std::vector<host::PlanStep> plan_steps;
   std::vector<char> plan_in_cover, plan_leads;
   std::vector<double> ov_host;
-  int ovl_mode = 0;  // overlap kernel: 0 by size, 1 vector FMA, 2 FP64 MFMA, 3 vector FMA in 48 x 48 tiles
+  int ovl_mode = 0;  // overlap kernel: 0 by size, 1 vector FMA, 2 FP64 MFMA, 3 vector FMA in 48 x 48 tiles,
+                    // 4 the same with every site sum Kahan-compensated
   bool table_pending = false;           // stage_ms[0] of the last table build not read yet
   size_t a_set = 0, b_set = 0, d_set = 0;  // elements of A, B, D per prepare
   bool xline = false;                // x passes on the line kernel, y passes skip empty lines
@@ -1446,7 +1447,7 @@ int clat_create(int X, int Y, int Z, int precision, int schedule, clat_engine** 
     if (const char* g = std::getenv("CLAT_PACK")) e->pack = std::atoi(g) != 0;
     if (const char* g = std::getenv("CLAT_OVL"))
       e->ovl_mode = std::strcmp(g, "valu") == 0 ? 1 : std::strcmp(g, "mfma") == 0 ? 2
-                    : std::strcmp(g, "valu48") == 0 ? 3 : 0;
+                    : std::strcmp(g, "valu48") == 0 ? 3 : std::strcmp(g, "kahan") == 0 ? 4 : 0;
     e->a_bytes = e->nb_max * e->a_set * e->s_a;
     e->A = e->dalloc(e->a_bytes);
     if (e->shared()) {
@@ -1799,7 +1800,8 @@ int clat_compute_integrals(clat_engine* e, const int32_t* quads, int64_t n, int 
     // 64 x 64 tiles on FP64 MFMA when both sides fill them; else 36 x 36
     // tiles on vector FMA when both sides fit one, 48 x 48 otherwise
     const bool mfma = e->ovl_mode == 2 || (e->ovl_mode == 0 && nL >= OVM_MIN && P >= OVM_MIN);
-    const bool t36 = !mfma && e->ovl_mode != 3 && nL <= 36 && P <= 36;
+    const bool kahan = e->ovl_mode == 4;  // 48 x 48 tiles, every site sum Kahan-compensated
+    const bool t36 = !mfma && !kahan && e->ovl_mode != 3 && nL <= 36 && P <= 36;
     const int OV_T = mfma ? OVM_T : t36 ? 36 : 48;
     const int ovs = mfma ? OVM_S : t36 ? 24 : OV_S;
     const int groups = t36 ? 3 : 1;  // site groups per block, a partial slice each
@@ -1839,6 +1841,9 @@ int clat_compute_integrals(clat_engine* e, const int32_t* quads, int64_t n, int 
       else if (t36)
         hipLaunchKernelGGL((k_overlap_partial<9, 4, 3, 24>), og, dim3(243), 0, e->stream, e->Vat, e->rho, nL, P,
                            e->n_sites, chunk, e->part);
+      else if (kahan)
+        hipLaunchKernelGGL((k_overlap_partial<16, 3, 1, OV_S, true>), og, dim3(256), 0, e->stream, e->Vat, e->rho,
+                           nL, P, e->n_sites, chunk, e->part);
       else
         hipLaunchKernelGGL((k_overlap_partial<16, 3, 1, OV_S>), og, dim3(256), 0, e->stream, e->Vat, e->rho, nL, P,
                            e->n_sites, chunk, e->part);

@@ -624,7 +624,18 @@ __global__ void k_products_direct(const double2* __restrict__ coeffs, int64_t n_
 // ---------------------------------------------------------------------------
 constexpr int OV_S = 16;   // sites staged per step (48 x 48 tiles)
 
-template <int TX, int PT, int G, int OVS>
+// s += a*b with Kahan compensation: c carries the negated low part lost so
+// far, so the compensated sum is s - c
+__device__ __forceinline__ void kahan_fma(double& s, double& c, double a, double b) {
+  const double y = fma(a, b, -c);
+  const double t = s + y;
+  c = (t - s) - y;
+  s = t;
+}
+
+// CMP: every one of the 4 PT^2 sums is Kahan-compensated site by site (a
+// second register per sum), so no sum over the sites is left uncompensated
+template <int TX, int PT, int G, int OVS, bool CMP = false>
 __global__ __launch_bounds__(TX * TX * G) void k_overlap_partial(const double2* __restrict__ Vat,
                                                                  const double2* __restrict__ rho,
                                                                  int n_left, int n_right, int64_t n_sites,
@@ -639,12 +650,16 @@ __global__ __launch_bounds__(TX * TX * G) void k_overlap_partial(const double2* 
   const int64_t s0 = (int64_t)blockIdx.z * chunk;
   const int64_t s1 = min(n_sites, s0 + chunk);
   double acc[PT][PT][4];
+  double cmp[CMP ? PT : 1][CMP ? PT : 1][4];
 #pragma unroll
   for (int i = 0; i < PT; ++i)
 #pragma unroll
     for (int j = 0; j < PT; ++j)
 #pragma unroll
-      for (int q = 0; q < 4; ++q) acc[i][j][q] = 0.0;
+      for (int q = 0; q < 4; ++q) {
+        acc[i][j][q] = 0.0;
+        if constexpr (CMP) cmp[i][j][q] = 0.0;
+      }
   constexpr int NE = (T * OVS + NTH - 1) / NTH;  // staged elements per thread and table
   // software pipeline: the next batch of sites is loaded into registers while
   // the current one is summed from LDS
@@ -682,10 +697,17 @@ __global__ __launch_bounds__(TX * TX * G) void k_overlap_partial(const double2* 
       for (int i = 0; i < PT; ++i)
 #pragma unroll
         for (int j = 0; j < PT; ++j) {
-          acc[i][j][0] = fma(v[i].x, r[j].x, acc[i][j][0]);
-          acc[i][j][1] = fma(v[i].y, r[j].y, acc[i][j][1]);
-          acc[i][j][2] = fma(v[i].x, r[j].y, acc[i][j][2]);
-          acc[i][j][3] = fma(v[i].y, r[j].x, acc[i][j][3]);
+          if constexpr (CMP) {
+            kahan_fma(acc[i][j][0], cmp[i][j][0], v[i].x, r[j].x);
+            kahan_fma(acc[i][j][1], cmp[i][j][1], v[i].y, r[j].y);
+            kahan_fma(acc[i][j][2], cmp[i][j][2], v[i].x, r[j].y);
+            kahan_fma(acc[i][j][3], cmp[i][j][3], v[i].y, r[j].x);
+          } else {
+            acc[i][j][0] = fma(v[i].x, r[j].x, acc[i][j][0]);
+            acc[i][j][1] = fma(v[i].y, r[j].y, acc[i][j][1]);
+            acc[i][j][2] = fma(v[i].x, r[j].y, acc[i][j][2]);
+            acc[i][j][3] = fma(v[i].y, r[j].x, acc[i][j][3]);
+          }
         }
     }
     __syncthreads();
@@ -699,7 +721,10 @@ __global__ __launch_bounds__(TX * TX * G) void k_overlap_partial(const double2* 
       if (l < n_left && r < n_right) {
         double* d = part + ((slice * n_left + l) * n_right + r) * 4;
 #pragma unroll
-        for (int q = 0; q < 4; ++q) d[q] = acc[i][j][q];
+        for (int q = 0; q < 4; ++q) {
+          if constexpr (CMP) d[q] = acc[i][j][q] - cmp[i][j][q];
+          else d[q] = acc[i][j][q];
+        }
       }
     }
 }

@@ -382,6 +382,11 @@ def test_batched_prepares(prec, monkeypatch):
     monkeypatch.setenv("CLAT_OVL", "valu48")
     on48 = run()
     assert _rel(on48, got) < 1e-12, _rel(on48, got)
+    # the same tiles with every site sum Kahan-compensated
+    monkeypatch.setenv("CLAT_OVL", "kahan")
+    onk = run()
+    assert _rel(onk, got) < 1e-12, _rel(onk, got)
+    check(onk, want, prec, quads)
 
 
 @pytest.mark.parametrize("prec", [clat.PREC_FP64, clat.PREC_FP32X], ids=["fp64", "fp32x"])
