@@ -204,6 +204,16 @@ def timed(eng, step, steps, warmup, dominant=False, breakdown=False):
         vals = step(eng)
     t_all = time.perf_counter() - t_all
     stats = eng.kernel_stats()
+    t_graph = None
+    if dominant:
+        # the same steps with kernel timing off: the device stage replays as
+        # HIP graphs (runs with timing events launch every pass directly)
+        eng.set_kernel_timing(False)
+        step(eng)
+        t_graph = time.perf_counter()
+        for _ in range(steps):
+            step(eng)
+        t_graph = time.perf_counter() - t_graph
     bd = None
     if breakdown:
         eng.set_kernel_timing(True)
@@ -213,7 +223,7 @@ def timed(eng, step, steps, warmup, dominant=False, breakdown=False):
         bd = {k: (v[0] / 2, v[1] / 2, v[2] / 2) for k, v in eng.kernel_stats().items()}
     eng.set_kernel_timing(False)
     return dict(vals=vals, total=t_all, steps=steps, dom=dom, stats=stats, breakdown=bd,
-                stage=eng.stage_ms().tolist())
+                stage=eng.stage_ms().tolist(), total_graph=t_graph, tuning=eng.tuning())
 
 
 def roofline_of(r, flops_overlap=None, traffic=None):
@@ -493,6 +503,8 @@ def main():
                    "integrals": len(quads), "schedule": args.schedule},
         "roofline": roof,
         "kernel_ms_per_step": {k: v[1] for k, v in head["breakdown"].items() if v[0]},
+        "graph_replay_ms_per_step": 1e3 * head["total_graph"] / args.steps,
+        "tuning": head["tuning"],
         "fft_pass_gbs": {k: (v[2] / max(1e-12, v[1] * 1e-3)) / 1e9 for k, v in head["breakdown"].items()
                          if v[0] and k in FFT_KINDS},
     }

@@ -41,7 +41,7 @@ SYMBOLS = [
     "clat_compute_potential_at_atoms_noalloc",
     "clat_last_stage_ms", "clat_set_kernel_timing", "clat_set_kernel_timing_kinds", "clat_kernel_kind_count",
     "clat_kernel_kind_name",
-    "clat_kernel_stats", "clat_reset_kernel_stats", "clat_engine_info",
+    "clat_kernel_stats", "clat_reset_kernel_stats", "clat_engine_info", "clat_set_tuning", "clat_tuning_info",
 ]
 
 
@@ -96,6 +96,9 @@ def lib() -> ctypes.CDLL:
                                     ctypes.POINTER(ctypes.c_double)]
     L.clat_reset_kernel_stats.argtypes = [vp]
     L.clat_engine_info.argtypes = [vp, _ip, ctypes.POINTER(_i64)]
+    L.clat_set_tuning.argtypes = [vp, ctypes.c_int]
+    L.clat_tuning_info.argtypes = [vp, ctypes.POINTER(ctypes.c_int), _dp, ctypes.POINTER(_i64),
+                                   ctypes.POINTER(_i64)]
     _LIB = L
     return L
 
@@ -356,6 +359,19 @@ class Engine:
             _check(lib().clat_kernel_stats(self.handle, k, ctypes.byref(n), ctypes.byref(ms), ctypes.byref(b)))
             out[name] = (n.value, ms.value, b.value)
         return out
+
+    def set_tuning(self, measure: bool) -> None:
+        _check(lib().clat_set_tuning(self.handle, 1 if measure else 0))
+
+    def tuning(self) -> dict:
+        """Prepares per launch in use, measured ms of 1/2/4/8, graph counts."""
+        b = ctypes.c_int()
+        ms = np.zeros(4)
+        rep = _i64()
+        cap = _i64()
+        _check(lib().clat_tuning_info(self.handle, ctypes.byref(b), ms, ctypes.byref(rep), ctypes.byref(cap)))
+        return {"batch": b.value, "trial_ms": {1 << i: float(ms[i]) for i in range(4) if ms[i] > 0},
+                "graph_replays": rep.value, "graph_captures": cap.value}
 
     def info(self):
         dim = np.zeros(3, np.int32)

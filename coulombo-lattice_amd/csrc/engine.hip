@@ -8,6 +8,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <array>
+#include <map>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -41,12 +42,16 @@ struct HipError {
   std::string msg;
 };
 
+// a failed call also clears the runtime's last-error state, so the failure
+// is reported once and later launch checks start clean
 #define HIPX(call)                                                                       \
   do {                                                                                   \
     hipError_t e_ = (call);                                                              \
-    if (e_ != hipSuccess)                                                                \
+    if (e_ != hipSuccess) {                                                              \
+      (void)hipGetLastError();                                                           \
       throw HipError{e_ == hipErrorOutOfMemory ? CLAT_ENOMEM : CLAT_EDEVICE,             \
                      std::string(#call) + ": " + hipGetErrorString(e_)};                 \
+    }                                                                                    \
   } while (0)
 
 #define LAUNCH_CHECK() HIPX(hipGetLastError())
@@ -270,6 +275,19 @@ struct clat_engine {
   const double2* b_in2[LINE_NB] = {};  // second real density packed into the prepare (or null)
   double2* b_out2[LINE_NB] = {};
   bool pack = true;                     // pairs of real (diagonal) densities share a prepare
+  // prepares per launch: measured on the first run of a shape (autotuner) or
+  // the fixed rule
+  int nb_rule = 1;
+  int tune = 1;                         // 1: measure (the default), 0: rule only
+  int tuned_nb = 0;                     // 0: not measured for the current atoms
+  double tune_ms[4] = {0, 0, 0, 0};     // prepare-stage ms of batches 1, 2, 4, 8 (0: not tried)
+  hipEvent_t tune_ev[9] = {};
+  int64_t atoms_gen = 0;                // bumped by every clat_set_atoms
+  // the device stage of a run (prepares, overlaps) replayed as one HIP graph
+  bool use_graph = true;
+  hipGraphExec_t gexec[2] = {};         // prepares, overlaps
+  std::vector<int64_t> gkey;            // what the graphs' kernel arguments depend on
+  int64_t graph_replays = 0, graph_captures = 0;
   int n_cu = 256;     // compute units of the device
   // host buffers of a run, kept with their pages between runs
   std::vector<host::PlanStep> plan_steps;
@@ -1176,6 +1194,273 @@ double2* scratch_of(int dev) {
   return g_scratch_ptr[dev];
 }
 
+// ===========================================================================
+// device stage of clat_compute_integrals: prepares of the cover, overlaps
+// ===========================================================================
+using Units = std::vector<std::array<int, 2>>;  // (slot, second real slot or -1) per prepare
+
+// Overlap launch shape: 64 x 64 tiles on FP64 MFMA when both sides fill
+// them; else 36 x 36 tiles on vector FMA when both sides fit one, 48 x 48
+// otherwise (or Kahan-compensated 48 x 48 when asked for)
+struct OvlCfg {
+  bool mfma = false, kahan = false, t36 = false;
+  int OV_T = 48, tl = 0, tr = 0, groups = 1;
+  int64_t n_chunks = 1, chunk = 1, n_slices = 1;
+  size_t n_out = 0;
+};
+
+OvlCfg overlap_cfg(const clat_engine* e, int nL, int P) {
+  OvlCfg c;
+  c.mfma = e->ovl_mode == 2 || (e->ovl_mode == 0 && nL >= OVM_MIN && P >= OVM_MIN);
+  c.kahan = !c.mfma && e->ovl_mode == 4;
+  c.t36 = !c.mfma && !c.kahan && e->ovl_mode != 3 && nL <= 36 && P <= 36;
+  c.OV_T = c.mfma ? OVM_T : c.t36 ? 36 : 48;
+  const int ovs = c.mfma ? OVM_S : c.t36 ? 24 : OV_S;
+  c.groups = c.t36 ? 3 : 1;  // site groups per block, a partial slice each
+  c.tl = (nL + c.OV_T - 1) / c.OV_T;
+  c.tr = (P + c.OV_T - 1) / c.OV_T;
+  // blocks: about 2048 for the 48 x 48 and MFMA tiles; for the 36 x 36
+  // tiles (246 VGPRs: two blocks per CU) one full round, two per CU
+  // (0.33 ms against 0.37 at 2.7 blocks per CU and 0.45 at 8, 1M atoms)
+  const int want_blocks = c.t36 ? 2 * e->n_cu : 2048;
+  int64_t n_chunks = std::max<int64_t>(1, want_blocks / std::max(1, c.tl * c.tr));
+  n_chunks = std::min<int64_t>(n_chunks, (e->n_sites + ovs - 1) / ovs);
+  n_chunks = std::max<int64_t>(1, n_chunks);
+  int64_t chunk = (e->n_sites + n_chunks - 1) / n_chunks;
+  chunk = std::max<int64_t>(ovs, (chunk + ovs - 1) / ovs * ovs);
+  c.chunk = chunk;
+  c.n_chunks = std::max<int64_t>(1, (e->n_sites + chunk - 1) / chunk);
+  c.n_slices = c.n_chunks * c.groups;
+  c.n_out = (size_t)std::max(1, nL) * P * 4;
+  return c;
+}
+
+void ensure_overlap_buffers(clat_engine* e, const OvlCfg& c) {
+  if (e->part_cap < c.n_out * c.n_slices) {
+    e->dfree(e->part, e->part_cap * sizeof(double));
+    e->part = (double*)e->dalloc(c.n_out * c.n_slices * sizeof(double));
+    e->part_cap = c.n_out * c.n_slices;
+  }
+  if (e->ovl_cap < c.n_out) {
+    e->dfree(e->ovl, e->ovl_cap * sizeof(double));
+    e->ovl = (double*)e->dalloc(c.n_out * sizeof(double));
+    e->ovl_cap = c.n_out;
+  }
+}
+
+// the (slot, product) overlap table into e->ovl (chunk partials, then their
+// compensated sum)
+void launch_overlaps(clat_engine* e, int nL, int P, const OvlCfg& c) {
+  e->kbegin(K_OVERLAP);
+  const dim3 og(c.tl, c.tr, (unsigned)c.n_chunks);
+  if (c.mfma)
+    hipLaunchKernelGGL(k_overlap_partial_mfma, og, dim3(256), 0, e->stream, e->Vat, e->rho, nL, P, e->n_sites,
+                       c.chunk, e->part);
+  else if (c.t36)
+    hipLaunchKernelGGL((k_overlap_partial<9, 4, 3, 24>), og, dim3(243), 0, e->stream, e->Vat, e->rho, nL, P,
+                       e->n_sites, c.chunk, e->part);
+  else if (c.kahan)
+    hipLaunchKernelGGL((k_overlap_partial<16, 3, 1, OV_S, true>), og, dim3(256), 0, e->stream, e->Vat, e->rho, nL,
+                       P, e->n_sites, c.chunk, e->part);
+  else
+    hipLaunchKernelGGL((k_overlap_partial<16, 3, 1, OV_S>), og, dim3(256), 0, e->stream, e->Vat, e->rho, nL, P,
+                       e->n_sites, c.chunk, e->part);
+  LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_overlap_reduce, dim3((unsigned)((c.n_out + 63) / 64)), dim3(1024), 0, e->stream, e->part,
+                     c.n_slices, (int64_t)c.n_out, e->ovl);
+  LAUNCH_CHECK();
+  e->kend(K_OVERLAP, (double)(c.tl * c.tr) * (double)(2 * c.OV_T) * (double)e->n_sites * 16.0);
+}
+
+// every prepare of the run, nbt per pass launch: pair density of slot
+// slots[s] -> potential of slot s at the sites (Vat)
+void run_prepares(clat_engine* e, const Units& units, const std::vector<int>& slots, int nbt) {
+  const int nU = (int)units.size();
+  for (int u = 0; u < nU;) {
+    const int nb = std::min(nbt, nU - u);
+    // the batch lives in engine state for the pass launches; the guard
+    // returns it to one plain prepare however this scope is left
+    BatchScope batch(e);
+    e->cur_nb = nb;
+    for (int b = 0; b < nb; ++b) {
+      const int s0 = units[(size_t)(u + b)][0], s1 = units[(size_t)(u + b)][1];
+      e->b_in[b] = e->rho + (size_t)slots[(size_t)s0] * e->n_sites;
+      e->b_out[b] = e->Vat + (size_t)s0 * e->n_sites;
+      e->b_in2[b] = s1 >= 0 ? e->rho + (size_t)slots[(size_t)s1] * e->n_sites : nullptr;
+      e->b_out2[b] = s1 >= 0 ? e->Vat + (size_t)s1 * e->n_sites : nullptr;
+    }
+    prepare_sites(e, e->b_in[0], e->b_out[0]);
+    u += nb;
+  }
+}
+
+// Measured batch sizes, shared by every engine of the process: the key is
+// what the pass launches depend on (grid, arithmetic, layout, occupied lines,
+// buffer capacity, device size).
+std::mutex g_tune_mu;
+std::map<std::vector<int64_t>, int> g_tuned;
+
+std::vector<int64_t> tune_key(const clat_engine* e) {
+  return {e->Xi, e->Yi, e->Zi, e->rprec, e->mix, e->sched, e->nb_max, e->n_live, e->n_sites,
+          e->atile ? 1 : 0, e->btile ? 1 : 0, e->n_cu};
+}
+
+// Prepares per pass launch for this run (the autotuner, the counterpart of
+// the reference's FFTW_MEASURE plans, CoulombCalculator.cpp:23,31).  Batches
+// of 1, 2, 4 and 8 prepares run the same kernels on the same data, so they
+// give bit-identical potentials and only their time differs.  On the first
+// run of a shape the whole prepare sequence is timed once per candidate, in
+// two rounds (forward and reverse order, after one untimed warm-up with the
+// rule's batch), and the fastest is kept for the engine and for the process.
+int choose_batch(clat_engine* e, const Units& units, const std::vector<int>& slots) {
+  if (!(e->xline && e->shared())) return 1;  // batching lives on the line-kernel path only
+  if (!e->tune) return e->nb_rule;
+  if (e->tuned_nb) return e->tuned_nb;
+  const std::vector<int64_t> key = tune_key(e);
+  {
+    std::lock_guard<std::mutex> lock(g_tune_mu);
+    auto it = g_tuned.find(key);
+    if (it != g_tuned.end()) return e->tuned_nb = it->second;
+  }
+  const int nU = (int)units.size();
+  std::vector<int> cand;
+  for (int c = 1; c <= e->nb_max; c *= 2)
+    if (c == 1 || c / 2 < nU) cand.push_back(c);
+  if (cand.size() == 1) return e->tuned_nb = cand[0];
+  for (auto& ev : e->tune_ev)
+    if (!ev) HIPX(hipEventCreate(&ev));
+  const uint32_t kmask = e->kmask;
+  e->kmask = 0;  // the trials stay out of the kernel statistics
+  std::vector<double> best(cand.size(), 1e30);
+  try {
+    run_prepares(e, units, slots, e->nb_rule);  // warm-up: code objects loaded, caches filled
+    for (int round = 0; round < 2; ++round) {
+      const int nc = (int)cand.size();
+      HIPX(hipEventRecord(e->tune_ev[0], e->stream));
+      for (int i = 0; i < nc; ++i) {
+        const int k = round ? nc - 1 - i : i;
+        run_prepares(e, units, slots, cand[(size_t)k]);
+        HIPX(hipEventRecord(e->tune_ev[i + 1], e->stream));
+      }
+      HIPX(hipEventSynchronize(e->tune_ev[nc]));
+      for (int i = 0; i < nc; ++i) {
+        const int k = round ? nc - 1 - i : i;
+        float ms = 0.f;
+        HIPX(hipEventElapsedTime(&ms, e->tune_ev[i], e->tune_ev[i + 1]));
+        best[(size_t)k] = std::min(best[(size_t)k], (double)ms);
+      }
+    }
+  } catch (...) {
+    e->kmask = kmask;
+    throw;
+  }
+  e->kmask = kmask;
+  size_t pick = 0;
+  for (size_t i = 0; i < cand.size(); ++i) {
+    if (best[i] < best[pick]) pick = i;
+    e->tune_ms[i] = best[i];
+  }
+  for (size_t i = cand.size(); i < 4; ++i) e->tune_ms[i] = 0.0;
+  e->tuned_nb = cand[pick];
+  std::lock_guard<std::mutex> lock(g_tune_mu);
+  g_tuned[key] = e->tuned_nb;
+  return e->tuned_nb;
+}
+
+// what the captured kernel arguments of a device stage depend on
+std::vector<int64_t> stage_key(const clat_engine* e, const Units& units, const std::vector<int>& slots, int nbt,
+                               const OvlCfg& c) {
+  std::vector<int64_t> k = {nbt, (int64_t)units.size(), (int64_t)slots.size(), (int64_t)(uintptr_t)e->rho,
+                            (int64_t)(uintptr_t)e->Vat, (int64_t)(uintptr_t)e->part, (int64_t)(uintptr_t)e->ovl,
+                            e->n_sites, e->atoms_gen, c.mfma, c.kahan, c.t36, c.tl, c.tr, c.n_chunks, c.chunk,
+                            (int64_t)c.n_out};
+  for (const auto& u : units) {
+    k.push_back(u[0]);
+    k.push_back(u[1]);
+  }
+  for (int s : slots) k.push_back(s);
+  return k;
+}
+
+void drop_graphs(clat_engine* e) {
+  for (auto& g : e->gexec)
+    if (g) {
+      (void)hipGraphExecDestroy(g);
+      g = nullptr;
+    }
+  e->gkey.clear();
+}
+
+// Capture what f launches on the engine stream into an executable graph;
+// nullptr when the runtime refuses (nothing is then left launched).
+template <typename F>
+hipGraphExec_t capture(clat_engine* e, F&& f) {
+  if (hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+    (void)hipGetLastError();
+    return nullptr;
+  }
+  try {
+    f();
+  } catch (...) {
+    hipGraph_t g = nullptr;
+    (void)hipStreamEndCapture(e->stream, &g);
+    if (g) (void)hipGraphDestroy(g);
+    (void)hipGetLastError();
+    throw;
+  }
+  hipGraph_t g = nullptr;
+  hipError_t st = hipStreamEndCapture(e->stream, &g);
+  hipGraphExec_t x = nullptr;
+  if (st == hipSuccess && g) st = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+  if (g) (void)hipGraphDestroy(g);
+  if (st != hipSuccess || !x) {
+    (void)hipGetLastError();
+    if (x) (void)hipGraphExecDestroy(x);
+    return nullptr;
+  }
+  return x;
+}
+
+// Prepares, the stage event, overlaps.  On the line-kernel path with kernel
+// timing off, the prepares and the overlaps are each captured once into a
+// HIP graph and replayed on later runs with the same plan and buffers (two
+// launches instead of about five per batch of prepares and two for the
+// overlaps); the stage event between them stays outside the graphs, whose
+// own event records the runtime cannot time.  A capture the runtime refuses
+// falls back to direct launches for good.
+void device_stage(clat_engine* e, const Units& units, const std::vector<int>& slots, int nbt, const OvlCfg& c) {
+  const int nL = (int)slots.size(), P = e->n_states * (e->n_states + 1) / 2;
+  auto prepares = [&] { run_prepares(e, units, slots, nbt); };
+  auto overlaps = [&] { launch_overlaps(e, nL, P, c); };
+  if (!(e->use_graph && e->kmask == 0 && e->xline && e->shared())) {
+    prepares();
+    HIPX(hipEventRecord(e->stage_ev[3], e->stream));
+    overlaps();
+    return;
+  }
+  const std::vector<int64_t> key = stage_key(e, units, slots, nbt, c);
+  if (!(e->gexec[0] && e->gexec[1] && key == e->gkey)) {
+    drop_graphs(e);
+    e->gexec[0] = capture(e, prepares);
+    e->gexec[1] = e->gexec[0] ? capture(e, overlaps) : nullptr;
+    if (!e->gexec[1]) {
+      drop_graphs(e);
+      e->use_graph = false;
+      prepares();
+      HIPX(hipEventRecord(e->stage_ev[3], e->stream));
+      overlaps();
+      return;
+    }
+    e->gkey = key;
+    ++e->graph_captures;
+  } else {
+    ++e->graph_replays;
+  }
+  HIPX(hipGraphLaunch(e->gexec[0], e->stream));
+  HIPX(hipEventRecord(e->stage_ev[3], e->stream));
+  HIPX(hipGraphLaunch(e->gexec[1], e->stream));
+}
+
 void free_atoms(clat_engine* e) {
   e->dfree(e->d_site_atom, (size_t)e->n_sites * 4);
   e->dfree(e->d_site_cell, (size_t)e->n_sites * 4);
@@ -1437,13 +1722,23 @@ int clat_create(int X, int Y, int Z, int precision, int schedule, clat_engine** 
     // eight when eight B planes take at most 1.1 GB (128^3: 57 against 60 ms
     // per batched-form step; 256x256x128 with eight: 10.4 against 10.2 ms)
     const size_t set_bytes = e->a_set * e->s_a + e->b_set * e->s_b + (e->s_d == e->s_a ? 0 : e->d_set * e->s_d);
+    // buffers for up to eight prepares per launch while the sets stay under
+    // 16 GB; which count runs is measured on the first run of the engine's
+    // shape (1, 2, 4 or 8 give bit-identical potentials).  Unmeasured, the
+    // rule is four, or eight when eight B planes take at most 1.1 GB (128^3:
+    // 57 against 60 ms per batched-form step; 256x256x128: 10.2 against 10.4)
     e->nb_max = 1;
     if (e->xline && e->conv_tiled) {
-      while (e->nb_max < 4 && (size_t)(2 * e->nb_max) * set_bytes <= ((size_t)16 << 30)) e->nb_max *= 2;
-      if (e->nb_max == 4 && 8 * e->b_set * e->s_b <= (size_t)1100 << 20) e->nb_max = 8;
+      while (e->nb_max < 8 && (size_t)(2 * e->nb_max) * set_bytes <= ((size_t)16 << 30)) e->nb_max *= 2;
+      e->nb_rule = std::min(e->nb_max, 8 * e->b_set * e->s_b <= (size_t)1100 << 20 ? 8 : 4);
     }
     static_assert(LINE_NB >= 8, "the pointer arrays of a launch hold eight prepares");
-    if (const char* g = std::getenv("CLAT_BATCH")) e->nb_max = std::max(1, std::min(e->nb_max, std::atoi(g)));
+    if (const char* g = std::getenv("CLAT_BATCH")) {
+      e->nb_max = e->nb_rule = std::max(1, std::min(e->nb_max, std::atoi(g)));
+      e->tune = 0;
+    }
+    if (const char* g = std::getenv("CLAT_TUNE")) e->tune = e->tune && std::atoi(g) != 0;
+    if (const char* g = std::getenv("CLAT_GRAPH")) e->use_graph = std::atoi(g) != 0;
     if (const char* g = std::getenv("CLAT_PACK")) e->pack = std::atoi(g) != 0;
     if (const char* g = std::getenv("CLAT_OVL"))
       e->ovl_mode = std::strcmp(g, "valu") == 0 ? 1 : std::strcmp(g, "mfma") == 0 ? 2
@@ -1484,6 +1779,10 @@ int clat_destroy(clat_engine* e) {
   for (auto ev : e->stage_ev)
     if (ev) (void)hipEventDestroy(ev);
   for (auto ev : e->kev) (void)hipEventDestroy(ev);
+  for (auto ev : e->tune_ev)
+    if (ev) (void)hipEventDestroy(ev);
+  for (auto g : e->gexec)
+    if (g) (void)hipGraphExecDestroy(g);
   if (e->stream) (void)hipStreamDestroy(e->stream);
   delete e;
   return CLAT_OK;
@@ -1667,6 +1966,8 @@ int clat_set_atoms(clat_engine* e, const int32_t* cell, int64_t n_atoms, int n_o
       }
     }
     free_atoms(e);  // also drops the loaded states (stored in the old site order)
+    e->atoms_gen += 1;  // a new geometry: new measured batch, new graph
+    e->tuned_nb = 0;
     e->n_atoms = n_atoms;
     e->n_orb = n_orbitals;
     e->n_sites = (int64_t)site_atom.size();
@@ -1778,80 +2079,23 @@ int clat_compute_integrals(clat_engine* e, const int32_t* quads, int64_t n, int 
       for (size_t i = 0; i < reals.size(); i += 2)
         units.push_back({reals[i], i + 1 < reals.size() ? reals[i + 1] : -1});
     }
-    const int nU = (int)units.size();
-    for (int u = 0; u < nU && e->n_sites > 0;) {
-      const int nb = std::min(e->nb_max, nU - u);
-      // the batch lives in engine state for the pass launches; the guard
-      // returns it to one plain prepare however this scope is left
-      BatchScope batch(e);
-      e->cur_nb = nb;
-      for (int b = 0; b < nb; ++b) {
-        const int s0 = units[(size_t)(u + b)][0], s1 = units[(size_t)(u + b)][1];
-        e->b_in[b] = e->rho + (size_t)slots[(size_t)s0] * e->n_sites;
-        e->b_out[b] = e->Vat + (size_t)s0 * e->n_sites;
-        e->b_in2[b] = s1 >= 0 ? e->rho + (size_t)slots[(size_t)s1] * e->n_sites : nullptr;
-        e->b_out2[b] = s1 >= 0 ? e->Vat + (size_t)s1 * e->n_sites : nullptr;
-      }
-      prepare_sites(e, e->b_in[0], e->b_out[0]);
-      u += nb;
-    }
-    HIPX(hipEventRecord(e->stage_ev[3], e->stream));
-    // overlaps of every (slot, product) pair
-    // 64 x 64 tiles on FP64 MFMA when both sides fill them; else 36 x 36
-    // tiles on vector FMA when both sides fit one, 48 x 48 otherwise
-    const bool mfma = e->ovl_mode == 2 || (e->ovl_mode == 0 && nL >= OVM_MIN && P >= OVM_MIN);
-    const bool kahan = e->ovl_mode == 4;  // 48 x 48 tiles, every site sum Kahan-compensated
-    const bool t36 = !mfma && !kahan && e->ovl_mode != 3 && nL <= 36 && P <= 36;
-    const int OV_T = mfma ? OVM_T : t36 ? 36 : 48;
-    const int ovs = mfma ? OVM_S : t36 ? 24 : OV_S;
-    const int groups = t36 ? 3 : 1;  // site groups per block, a partial slice each
-    const int tl = (nL + OV_T - 1) / OV_T, tr = (P + OV_T - 1) / OV_T;
-    // blocks: about 2048 for the 48 x 48 and MFMA tiles; for the 36 x 36
-    // tiles (246 VGPRs: two blocks per CU) one full round, two per CU
-    // (0.33 ms against 0.37 at 2.7 blocks per CU and 0.45 at 8, 1M atoms)
-    const int want_blocks = t36 ? 2 * e->n_cu : 2048;
-    int64_t n_chunks = std::max<int64_t>(1, want_blocks / std::max(1, tl * tr));
-    n_chunks = std::min<int64_t>(n_chunks, (e->n_sites + ovs - 1) / ovs);
-    n_chunks = std::max<int64_t>(1, n_chunks);
-    int64_t chunk = (e->n_sites + n_chunks - 1) / n_chunks;
-    chunk = std::max<int64_t>(ovs, (chunk + ovs - 1) / ovs * ovs);
-    n_chunks = std::max<int64_t>(1, (e->n_sites + chunk - 1) / chunk);
-    const int64_t n_slices = n_chunks * groups;
-    const size_t n_out = (size_t)std::max(1, nL) * P * 4;
-    if (e->part_cap < n_out * n_slices) {
-      e->dfree(e->part, e->part_cap * sizeof(double));
-      e->part = (double*)e->dalloc(n_out * n_slices * sizeof(double));
-      e->part_cap = n_out * n_slices;
-    }
-    if (e->ovl_cap < n_out) {
-      e->dfree(e->ovl, e->ovl_cap * sizeof(double));
-      e->ovl = (double*)e->dalloc(n_out * sizeof(double));
-      e->ovl_cap = n_out;
-    }
+    // overlap launch shape; its buffers are allocated here, before any
+    // capture of the device stage
+    const OvlCfg oc = overlap_cfg(e, nL, P);
+    ensure_overlap_buffers(e, oc);
+    const size_t n_out = oc.n_out;
     std::vector<double>& ov = e->ov_host;  // kept between runs, like the plan buffer
     ov.resize(n_out);
     // with no sites every overlap is zero (the buffer holds the last run's)
-    if (!(nL > 0 && e->n_sites > 0)) std::fill(ov.begin(), ov.end(), 0.0);
-    if (nL > 0 && e->n_sites > 0) {
-      e->kbegin(K_OVERLAP);
-      const dim3 og(tl, tr, (unsigned)n_chunks);
-      if (mfma)
-        hipLaunchKernelGGL(k_overlap_partial_mfma, og, dim3(256), 0, e->stream, e->Vat, e->rho, nL, P, e->n_sites,
-                           chunk, e->part);
-      else if (t36)
-        hipLaunchKernelGGL((k_overlap_partial<9, 4, 3, 24>), og, dim3(243), 0, e->stream, e->Vat, e->rho, nL, P,
-                           e->n_sites, chunk, e->part);
-      else if (kahan)
-        hipLaunchKernelGGL((k_overlap_partial<16, 3, 1, OV_S, true>), og, dim3(256), 0, e->stream, e->Vat, e->rho,
-                           nL, P, e->n_sites, chunk, e->part);
-      else
-        hipLaunchKernelGGL((k_overlap_partial<16, 3, 1, OV_S>), og, dim3(256), 0, e->stream, e->Vat, e->rho, nL, P,
-                           e->n_sites, chunk, e->part);
-      LAUNCH_CHECK();
-      hipLaunchKernelGGL(k_overlap_reduce, dim3((unsigned)((n_out + 63) / 64)), dim3(1024), 0, e->stream, e->part,
-                         n_slices, (int64_t)n_out, e->ovl);
-      LAUNCH_CHECK();
-      e->kend(K_OVERLAP, (double)(tl * tr) * (double)(2 * OV_T) * (double)e->n_sites * 16.0);
+    if (!(nL > 0 && e->n_sites > 0)) {
+      std::fill(ov.begin(), ov.end(), 0.0);
+      HIPX(hipEventRecord(e->stage_ev[3], e->stream));
+    } else {
+      // prepares per launch: measured on the first run of this shape (the
+      // choices give bit-identical potentials), then the prepares and the
+      // overlaps as one replayed graph where it can be captured
+      const int nbt = choose_batch(e, units, slots);
+      device_stage(e, units, slots, nbt, oc);
     }
     HIPX(hipEventRecord(e->stage_ev[4], e->stream));
     // the steps, on the host while the queued prepares and overlaps run (the
@@ -1998,6 +2242,23 @@ int clat_engine_info(clat_engine* e, int dim[3], int64_t* device_bytes) {
   if (!e) return set_err(CLAT_EINVAL, "invalid arguments");
   if (dim) { dim[0] = e->Xi; dim[1] = e->Yi; dim[2] = e->Zi; }
   if (device_bytes) *device_bytes = e->device_bytes;
+  return CLAT_OK;
+}
+
+int clat_set_tuning(clat_engine* e, int measure) {
+  if (!e) return set_err(CLAT_EINVAL, "invalid arguments");
+  e->tune = measure ? 1 : 0;
+  return CLAT_OK;
+}
+
+int clat_tuning_info(clat_engine* e, int* batch, double trial_ms[4], int64_t* graph_replays,
+                     int64_t* graph_captures) {
+  if (!e) return set_err(CLAT_EINVAL, "invalid arguments");
+  if (batch) *batch = e->tune ? e->tuned_nb : e->nb_rule;
+  if (trial_ms)
+    for (int i = 0; i < 4; ++i) trial_ms[i] = e->tune_ms[i];
+  if (graph_replays) *graph_replays = e->graph_replays;
+  if (graph_captures) *graph_captures = e->graph_captures;
   return CLAT_OK;
 }
 

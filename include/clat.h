@@ -229,6 +229,31 @@ int clat_reset_kernel_stats(clat_engine* e);
  * bytes it currently holds. */
 int clat_engine_info(clat_engine* e, int dim[3], int64_t* device_bytes);
 
+/* ---- planning ---------------------------------------------------------- */
+
+/* The measureFFT choice of CoulombCalculator(const Dimension&, bool
+ * measureFFT = true) (CoulombCalculator.hpp:57, CoulombCalculator.cpp:23,31),
+ * for this engine's own plans.  measure = 1 (the default, as in the
+ * reference): the first clat_compute_integrals of an atom list times the
+ * whole prepare sequence with 1, 2, 4 and 8 prepares per pass launch and
+ * keeps the fastest, for the engine and for every later engine of the process
+ * with the same grid, arithmetic and occupied lines.  The counts give
+ * bit-identical results; the first run takes a few prepare sequences longer.
+ * measure = 0: a fixed rule (four, or eight on grids whose eight
+ * intermediate planes take at most 1.1 GB).  CLAT_TUNE=0 in the environment
+ * at clat_create turns measuring off; CLAT_BATCH=n fixes the count. */
+int clat_set_tuning(clat_engine* e, int measure);
+
+/* The prepares per launch in use (0: not measured yet), the measured
+ * prepare-sequence milliseconds of 1, 2, 4 and 8 (0: not tried), and how
+ * often the device stage of clat_compute_integrals was captured into and
+ * replayed from HIP graphs (runs with an unchanged plan and buffers replay
+ * two graphs, prepares and overlaps, instead of launching every pass; runs
+ * with kernel timing on, and every run after CLAT_GRAPH=0 at clat_create,
+ * launch directly).  No counterpart in the reference. */
+int clat_tuning_info(clat_engine* e, int* batch, double trial_ms[4], int64_t* graph_replays,
+                     int64_t* graph_captures);
+
 #ifdef __cplusplus
 }
 #endif

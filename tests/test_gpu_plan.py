@@ -1,0 +1,113 @@
+"""The engine's own planning: measured prepares per launch (the counterpart of
+the reference's FFTW_MEASURE plans, CoulombCalculator.cpp:23,31) and the
+device stage replayed as one HIP graph.  Neither may change a single bit of
+the integrals."""
+import numpy as np
+import pytest
+
+import clat
+import zincblende as zb
+
+pytestmark = pytest.mark.gpu
+
+
+def _engine(w, prec, n_orb=4):
+    X, Y, Z = w["dims"]
+    eng = clat.Engine((X, Y, Z), prec, clat.SCHED_SHARED)
+    eng.initialize(clat.InteractionSimple(w["step"], 0.0, 10.89))
+    eng.set_atoms(w["cells"], n_orb)
+    for s in w["states"]:
+        eng.add_state(s)
+    return eng
+
+
+@pytest.mark.parametrize("prec", [clat.PREC_FP64, clat.PREC_FP32X], ids=["fp64", "fp32x"])
+def test_graph_replays_match_direct_launches(prec, monkeypatch):
+    """Runs 2.. of one plan replay the captured graph.  Between the replays
+    the engine does eager work of its own (a potential, a run with another
+    integral list, which re-captures, a new interaction table) and every
+    replay must give exactly the values of direct launches."""
+    print("case graph replay", prec, flush=True)
+    w = zb.workload(8, 8, 8, 3, 2, config=0, sigma_frac=0.2, n_orb=4)
+    quads = clat.pattern_enumerate("****", 5, 3)
+    other = clat.pattern_enumerate("eeee,hhhh", 5, 3)
+    monkeypatch.setenv("CLAT_GRAPH", "0")
+    ref_eng = _engine(w, prec)
+    ref = ref_eng.compute_integrals(quads)
+    ref_other = ref_eng.compute_integrals(other)
+    ref_pot = ref_eng.potential_at_atoms(2)
+    assert ref_eng.tuning()["graph_captures"] == 0
+    ref_eng.close()
+    monkeypatch.delenv("CLAT_GRAPH")
+    eng = _engine(w, prec)
+    for k in range(4):
+        assert np.array_equal(eng.compute_integrals(quads), ref), k
+        assert np.array_equal(eng.potential_at_atoms(2), ref_pot)
+        if k == 1:
+            assert np.array_equal(eng.compute_integrals(other), ref_other)
+        if k == 2:
+            eng.initialize(clat.InteractionSimple(w["step"], 0.0, 10.89))
+    t = eng.tuning()
+    assert t["graph_captures"] >= 2 and t["graph_replays"] >= 2, t
+    eng.close()
+
+
+def test_kernel_timing_runs_launch_directly(monkeypatch):
+    """Runs with kernel timing on launch every pass directly (events inside a
+    graph cannot be timed on this runtime): the per-kind launch counts and
+    timings are those of an engine without graphs, no replay happens while
+    timing is on, and replays resume when it is off."""
+    print("case graph timing", flush=True)
+    w = zb.workload(8, 8, 8, 2, 2, config=0, sigma_frac=0.2, n_orb=4)
+    quads = clat.pattern_enumerate("****", 4, 2)
+    counts = {}
+    for graph in ("0", "1"):
+        monkeypatch.setenv("CLAT_GRAPH", graph)
+        eng = _engine(w, clat.PREC_FP32X)
+        eng.compute_integrals(quads)  # measures the batch outside the statistics
+        eng.set_kernel_timing(True)
+        eng.reset_kernel_stats()
+        for _ in range(4):
+            eng.compute_integrals(quads)
+        st = eng.kernel_stats()
+        counts[graph] = {k: v[0] for k, v in st.items()}
+        assert all(v[1] > 0 for v in st.values() if v[0]), st
+        assert eng.tuning()["graph_replays"] == 0
+        if graph == "1":
+            eng.set_kernel_timing(False)
+            for _ in range(3):
+                eng.compute_integrals(quads)
+            # the graphs captured by the untimed first run replay again
+            assert eng.tuning()["graph_replays"] == 3
+        eng.close()
+    assert counts["0"] == counts["1"] and counts["1"]["zconv"] > 0
+
+
+def test_measured_batch_is_cached_and_bit_identical(monkeypatch):
+    """The first run of a shape times the candidate batches; a second engine
+    of the same shape takes the cached choice without trials; a rule-only
+    engine and one prepare per launch give the same bits.  (The choice is
+    cached per process, so this shape, a 32x32x64 grid, is used here only.)"""
+    print("case autotuner", flush=True)
+    w = zb.workload(8, 8, 16, 3, 2, config=0, sigma_frac=0.2, n_orb=4)
+    quads = clat.pattern_enumerate("****", 5, 3)
+    a = _engine(w, clat.PREC_FP32X)
+    va = a.compute_integrals(quads)
+    ta = a.tuning()
+    assert ta["batch"] in (1, 2, 4, 8) and len(ta["trial_ms"]) >= 2, ta
+    assert ta["batch"] == min(ta["trial_ms"], key=ta["trial_ms"].get)
+    b = _engine(w, clat.PREC_FP32X)
+    vb = b.compute_integrals(quads)
+    tb = b.tuning()
+    assert tb["batch"] == ta["batch"] and tb["trial_ms"] == {}, tb
+    c = _engine(w, clat.PREC_FP32X)
+    c.set_tuning(False)
+    vc = c.compute_integrals(quads)
+    monkeypatch.setenv("CLAT_BATCH", "1")
+    d = _engine(w, clat.PREC_FP32X)
+    vd = d.compute_integrals(quads)
+    assert d.tuning()["batch"] == 1
+    for v in (vb, vc, vd):
+        assert np.array_equal(v, va)
+    for e in (a, b, c, d):
+        e.close()
