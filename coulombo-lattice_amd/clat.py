@@ -37,7 +37,8 @@ SYMBOLS = [
     "clat_pattern_enumerate", "clat_vertex_cover", "clat_plan", "clat_create", "clat_destroy",
     "clat_set_interaction_simple", "clat_set_interaction_thomas_fermi", "clat_set_interaction_table",
     "clat_set_input", "clat_prepare", "clat_calculate", "clat_get_potential", "clat_set_atoms",
-    "clat_add_state", "clat_clear_states", "clat_compute_integrals", "clat_compute_potential_at_atoms",
+    "clat_add_state", "clat_clear_states", "clat_compute_integrals", "clat_compute_integrals_group",
+    "clat_compute_potential_at_atoms",
     "clat_compute_potential_at_atoms_noalloc",
     "clat_last_stage_ms", "clat_set_kernel_timing", "clat_set_kernel_timing_kinds", "clat_kernel_kind_count",
     "clat_kernel_kind_name",
@@ -84,6 +85,7 @@ def lib() -> ctypes.CDLL:
     L.clat_add_state.argtypes = [vp, _dp]
     L.clat_clear_states.argtypes = [vp]
     L.clat_compute_integrals.argtypes = [vp, vp, _i64, ctypes.c_int, vp]
+    L.clat_compute_integrals_group.argtypes = [vp, ctypes.c_int, vp, vp, ctypes.c_int, vp]
     L.clat_compute_potential_at_atoms.argtypes = [vp, ctypes.c_int, _dp]
     L.clat_compute_potential_at_atoms_noalloc.argtypes = [vp, ctypes.c_int, _dp]
     L.clat_last_stage_ms.argtypes = [vp, _dp]
@@ -383,11 +385,33 @@ class Engine:
         self._h.close()
 
 
+def compute_integrals_group(engines, quad_lists, stale_values: bool = False) -> list:
+    """Integrals of several systems (one engine each, any grid sizes) with
+    their device stages launched as one HIP graph; returns one complex array
+    per engine, bit-identical to each engine's compute_integrals."""
+    engines = list(engines)
+    qs = [np.ascontiguousarray(np.asarray(q, np.int32).reshape(-1, 4)) for q in quad_lists]
+    if len(qs) != len(engines) or not engines:
+        raise ValueError("one integral list per engine, at least one engine")
+    outs = [np.zeros(len(q), np.complex128) for q in qs]
+    k = len(engines)
+    hs = (ctypes.c_void_p * k)(*[e.handle for e in engines])
+    qp = (ctypes.c_void_p * k)(*[q.ctypes.data if len(q) else None for q in qs])
+    ns = (_i64 * k)(*[len(q) for q in qs])
+    op = (ctypes.c_void_p * k)(*[o.ctypes.data if len(o) else None for o in outs])
+    _check(lib().clat_compute_integrals_group(ctypes.cast(hs, ctypes.c_void_p), k, ctypes.cast(qp, ctypes.c_void_p),
+                                              ctypes.cast(ns, ctypes.c_void_p),
+                                              FLAG_STALE_VALUES if stale_values else 0,
+                                              ctypes.cast(op, ctypes.c_void_p)))
+    return outs
+
+
 def run_concurrent(jobs) -> list:
     """Run several jobs at once, one host thread each, and return their results
     in order.  Each job is a callable without arguments that uses engines of
     its own (an engine handle is single-threaded).  Every engine launches on
-    its own stream and the library holds no shared state, so the GPU overlaps
+    its own stream and shares only the tuner's choices (under a lock) and the
+    allocation-free entry point's array with other engines, so the GPU overlaps
     the engines' kernels: the passes of small grids fill the device while a
     large grid streams.  ctypes releases the interpreter lock during each
     library call."""

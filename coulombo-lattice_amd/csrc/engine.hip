@@ -8,6 +8,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <array>
+#include <atomic>
 #include <map>
 #include <mutex>
 #include <string>
@@ -286,8 +287,12 @@ struct clat_engine {
   // the device stage of a run (prepares, overlaps) replayed as one HIP graph
   bool use_graph = true;
   hipGraphExec_t gexec[2] = {};         // prepares, overlaps
+  hipGraphExec_t ggroup = nullptr;      // device stages of a group this engine leads
+  std::vector<int64_t> ggroup_key;
+  hipEvent_t join_ev = nullptr;         // stream joins of group runs
   std::vector<int64_t> gkey;            // what the graphs' kernel arguments depend on
   int64_t graph_replays = 0, graph_captures = 0;
+  int64_t serial = 0;                   // unique per engine of the process (group graph keys)
   int n_cu = 256;     // compute units of the device
   // host buffers of a run, kept with their pages between runs
   std::vector<host::PlanStep> plan_steps;
@@ -1461,6 +1466,209 @@ void device_stage(clat_engine* e, const Units& units, const std::vector<int>& sl
   HIPX(hipGraphLaunch(e->gexec[1], e->stream));
 }
 
+// ===========================================================================
+// one integral run, in phases (a group of engines shares the middle one)
+// ===========================================================================
+struct RunPlan {
+  int P = 0, nL = 0, nbt = 1;
+  bool stale = false, device = false;
+  std::vector<int> slot_of, slots;
+  Units units;
+  OvlCfg oc;
+};
+
+// Pair densities (queued first: they run while the host plans), the cover,
+// the left-product slots, the prepare units, the buffers and the batch.
+void run_begin(clat_engine* e, const int32_t* quads, int64_t n, int flags, RunPlan& rp) {
+  HIPX(hipEventRecord(e->stage_ev[1], e->stream));
+  e->rho_states = -1;  // the pair densities are part of every run's compute stage
+  build_products(e);
+  HIPX(hipEventRecord(e->stage_ev[2], e->stream));
+  // the stale-values bookkeeping walks the steps in execution order; without
+  // it every value depends on its own step only and id order serves
+  rp.stale = (flags & CLAT_FLAG_STALE_VALUES) != 0;
+  // the planner's cover fixes which products lead steps and so the prepare
+  // order; the steps themselves are built while the prepares run
+  host::plan_cover(quads, n, e->n_states, e->plan_in_cover, e->plan_leads);
+  rp.P = e->n_states * (e->n_states + 1) / 2;
+  // left products in execution order (ascending product index) -> slots
+  rp.slot_of.assign((size_t)rp.P, -1);
+  for (int p = 0; p < rp.P; ++p)
+    if (e->plan_leads[(size_t)p]) {
+      rp.slot_of[(size_t)p] = (int)rp.slots.size();
+      rp.slots.push_back(p);
+    }
+  rp.nL = (int)rp.slots.size();
+  const size_t vat_need = (size_t)std::max(1, rp.nL) * (size_t)std::max<int64_t>(1, e->n_sites);
+  if (e->vat_cap < vat_need) {
+    e->dfree(e->Vat, e->vat_cap * sizeof(double2));
+    e->Vat = (double2*)e->dalloc(vat_need * sizeof(double2));
+    e->vat_cap = vat_need;
+  }
+  // prepare units: on the line-kernel path two real (diagonal) densities
+  // share one complex prepare, their potentials being its real and
+  // imaginary parts (the kernel is real)
+  {
+    std::vector<int> reals;
+    const bool pk = e->pack && e->xline && e->shared();
+    for (int sl = 0; sl < rp.nL; ++sl) {
+      int L = 0;
+      while ((L + 1) * (L + 2) / 2 <= rp.slots[(size_t)sl]) ++L;
+      const bool diag = rp.slots[(size_t)sl] == L * (L + 1) / 2 + L;
+      if (pk && diag) reals.push_back(sl);
+      else rp.units.push_back({sl, -1});
+    }
+    for (size_t i = 0; i < reals.size(); i += 2)
+      rp.units.push_back({reals[i], i + 1 < reals.size() ? reals[i + 1] : -1});
+  }
+  // overlap launch shape; its buffers are allocated here, before any
+  // capture of the device stage
+  rp.oc = overlap_cfg(e, rp.nL, rp.P);
+  ensure_overlap_buffers(e, rp.oc);
+  e->ov_host.resize(rp.oc.n_out);  // kept between runs, like the plan buffer
+  rp.device = rp.nL > 0 && e->n_sites > 0;
+  // with no sites every overlap is zero (the buffer holds the last run's)
+  if (!rp.device) std::fill(e->ov_host.begin(), e->ov_host.end(), 0.0);
+  // prepares per launch: measured on the first run of this shape (the
+  // choices give bit-identical potentials)
+  else rp.nbt = choose_batch(e, rp.units, rp.slots);
+}
+
+// The steps (on the host while the device stage runs), the overlap table to
+// the host, the main loop's bookkeeping (coulombo.cpp:186-209), stage times.
+void run_finish(clat_engine* e, const int32_t* quads, int64_t n, const RunPlan& rp, double* out) {
+  HIPX(hipEventRecord(e->stage_ev[4], e->stream));
+  // the copy of the overlap table to pageable memory below waits for the
+  // queued prepares and overlaps
+  host::make_plan_into(quads, n, e->n_states, rp.stale, e->plan_steps, &e->plan_in_cover);
+  const std::vector<host::PlanStep>& steps = e->plan_steps;
+  std::vector<double>& ov = e->ov_host;
+  if (rp.device)
+    HIPX(hipMemcpyAsync(ov.data(), e->ovl, rp.oc.n_out * sizeof(double), hipMemcpyDeviceToHost, e->stream));
+  HIPX(hipStreamSynchronize(e->stream));
+  e->kcollect();
+  const int P = rp.P;
+  auto value_of = [&](const host::PlanStep& s, int rc, double& vre, double& vim) {
+    const double* q = ov.data() + (((size_t)rp.slot_of[(size_t)s.left.product]) * P + s.right.product) * 4;
+    const double Av = q[0], Bv = q[1], Cv = q[2], Dv = q[3];
+    if (rc) { vre = Av + Bv; vim = Dv - Cv; }
+    else { vre = Av - Bv; vim = Cv + Dv; }
+  };
+  if (!rp.stale) {
+    // every value from its own step: host threads over id ranges
+    host::parallel_ranges(steps.size(), [&](size_t k0, size_t k1, size_t) {
+      for (size_t k = k0; k < k1; ++k) {
+        const auto& s = steps[k];
+        double vre, vim;
+        value_of(s, (s.left.conj != s.right.conj) ? 1 : 0, vre, vim);
+        out[2 * (size_t)s.id] = vre;
+        out[2 * (size_t)s.id + 1] = s.left.conj ? -vim : vim;
+      }
+    });
+  } else {
+    int lastR = -1, lastRc = 0;
+    double vre = 0.0, vim = 0.0;
+    for (const auto& s : steps) {
+      const int rc = (s.left.conj != s.right.conj) ? 1 : 0;
+      if (s.right.product != lastR || rc != lastRc) {
+        value_of(s, rc, vre, vim);
+        lastR = s.right.product;
+        lastRc = rc;
+      }
+      out[2 * (size_t)s.id] = vre;
+      out[2 * (size_t)s.id + 1] = s.left.conj ? -vim : vim;
+    }
+  }
+  float ms;
+  HIPX(hipEventElapsedTime(&ms, e->stage_ev[1], e->stage_ev[2])); e->stage_ms[1] = ms;
+  HIPX(hipEventElapsedTime(&ms, e->stage_ev[2], e->stage_ev[3])); e->stage_ms[2] = ms;
+  HIPX(hipEventElapsedTime(&ms, e->stage_ev[3], e->stage_ev[4])); e->stage_ms[3] = ms;
+  HIPX(hipEventElapsedTime(&ms, e->stage_ev[1], e->stage_ev[4])); e->stage_ms[4] = ms;
+}
+
+// The device stages of several engines (systems of any grid sizes) as one
+// HIP graph on the first engine's stream: captured once, then launched once
+// per group run while every engine's plan and buffers stay the same.  The
+// first engine's stream waits for every engine's pair densities first, and
+// every engine's stream waits for the graph before its overlap table is
+// copied.  With kernel timing on, or when the runtime refuses the capture,
+// each engine runs its own device stage on its own stream instead.  A group
+// run reports its whole device stage as every member's prepare stage.
+void group_stage(clat_engine* const* es, int count, std::vector<RunPlan>& rps) {
+  clat_engine* lead = es[0];
+  bool graph = lead->use_graph;
+  for (int i = 0; i < count; ++i) {
+    const clat_engine* e = es[i];
+    graph = graph && e->use_graph && e->kmask == 0 && (!rps[(size_t)i].device || (e->xline && e->shared()));
+  }
+  auto each_direct = [&] {
+    for (int i = 0; i < count; ++i) {
+      clat_engine* e = es[i];
+      const RunPlan& rp = rps[(size_t)i];
+      if (rp.device) device_stage(e, rp.units, rp.slots, rp.nbt, rp.oc);
+      else HIPX(hipEventRecord(e->stage_ev[3], e->stream));
+    }
+  };
+  if (!graph) return each_direct();
+  for (int i = 0; i < count; ++i)
+    if (!es[i]->join_ev) HIPX(hipEventCreateWithFlags(&es[i]->join_ev, hipEventDisableTiming));
+  // the graph's key: every member and what its captured arguments depend on
+  std::vector<int64_t> key;
+  for (int i = 0; i < count; ++i) {
+    const RunPlan& rp = rps[(size_t)i];
+    key.push_back(es[i]->serial);  // not the address: a later engine may reuse it
+    key.push_back(rp.device ? 1 : 0);
+    if (rp.device) {
+      const std::vector<int64_t> k = stage_key(es[i], rp.units, rp.slots, rp.nbt, rp.oc);
+      key.push_back((int64_t)k.size());
+      key.insert(key.end(), k.begin(), k.end());
+    }
+  }
+  hipStream_t gs = lead->stream;
+  for (int i = 1; i < count; ++i) {
+    HIPX(hipEventRecord(es[i]->join_ev, es[i]->stream));
+    HIPX(hipStreamWaitEvent(gs, es[i]->join_ev, 0));
+  }
+  if (!(lead->ggroup && key == lead->ggroup_key)) {
+    if (lead->ggroup) {
+      (void)hipGraphExecDestroy(lead->ggroup);
+      lead->ggroup = nullptr;
+      lead->ggroup_key.clear();
+    }
+    auto all = [&] {
+      for (int i = 0; i < count; ++i) {
+        clat_engine* e = es[i];
+        const RunPlan& rp = rps[(size_t)i];
+        if (!rp.device) continue;
+        // this member's launches go to the lead stream for the capture
+        struct StreamSwap {
+          clat_engine* e;
+          hipStream_t own;
+          ~StreamSwap() { e->stream = own; }
+        } swap{e, e->stream};
+        e->stream = gs;
+        run_prepares(e, rp.units, rp.slots, rp.nbt);
+        launch_overlaps(e, rp.nL, rp.P, rp.oc);
+      }
+    };
+    lead->ggroup = capture(lead, all);
+    if (!lead->ggroup) {
+      lead->use_graph = false;
+      return each_direct();
+    }
+    lead->ggroup_key = key;
+    ++lead->graph_captures;
+  } else {
+    ++lead->graph_replays;
+  }
+  HIPX(hipGraphLaunch(lead->ggroup, gs));
+  HIPX(hipEventRecord(lead->join_ev, gs));
+  for (int i = 0; i < count; ++i) {
+    if (i > 0) HIPX(hipStreamWaitEvent(es[i]->stream, lead->join_ev, 0));
+    HIPX(hipEventRecord(es[i]->stage_ev[3], es[i]->stream));
+  }
+}
+
 void free_atoms(clat_engine* e) {
   e->dfree(e->d_site_atom, (size_t)e->n_sites * 4);
   e->dfree(e->d_site_cell, (size_t)e->n_sites * 4);
@@ -1568,6 +1776,8 @@ int clat_create(int X, int Y, int Z, int precision, int schedule, clat_engine** 
   *out = nullptr;
   clat_engine* e = new (std::nothrow) clat_engine();
   if (!e) return set_err(CLAT_ENOMEM, "host allocation failed");
+  static std::atomic<int64_t> next_serial{1};
+  e->serial = next_serial++;
   int rc = guarded([&] {
     if (X <= 0 || Y <= 0 || Z <= 0) throw host::Error(CLAT_EINVAL, "dimensions must be positive");
     if ((double)X * Y * Z >= 2147483647.0) throw host::Error(CLAT_EINVAL, "dimensions are too large");
@@ -1783,6 +1993,8 @@ int clat_destroy(clat_engine* e) {
     if (ev) (void)hipEventDestroy(ev);
   for (auto g : e->gexec)
     if (g) (void)hipGraphExecDestroy(g);
+  if (e->ggroup) (void)hipGraphExecDestroy(e->ggroup);
+  if (e->join_ev) (void)hipEventDestroy(e->join_ev);
   if (e->stream) (void)hipStreamDestroy(e->stream);
   delete e;
   return CLAT_OK;
@@ -2035,114 +2247,31 @@ int clat_compute_integrals(clat_engine* e, const int32_t* quads, int64_t n, int 
   if (!e->initialized) return set_err(CLAT_ESTATE, "calculator is not initialized");
   if (e->n_states <= 0) return set_err(CLAT_ESTATE, "no states loaded");
   return guarded([&] {
-    // the pair densities do not depend on the plan: queued first, they run
-    // while the host plans
-    HIPX(hipEventRecord(e->stage_ev[1], e->stream));
-    e->rho_states = -1;  // the pair densities are part of every run's compute stage
-    build_products(e);
-    HIPX(hipEventRecord(e->stage_ev[2], e->stream));
-    // the stale-values bookkeeping walks the steps in execution order; without
-    // it every value depends on its own step only and id order serves
-    const bool stale = (flags & CLAT_FLAG_STALE_VALUES) != 0;
-    // the planner's cover fixes which products lead steps and so the prepare
-    // order; the steps themselves are built while the prepares run
-    host::plan_cover(quads, n, e->n_states, e->plan_in_cover, e->plan_leads);
-    const int P = e->n_states * (e->n_states + 1) / 2;
-    // left products in execution order (ascending product index) -> slots
-    std::vector<int> slot_of((size_t)P, -1), slots;
-    for (int p = 0; p < P; ++p)
-      if (e->plan_leads[(size_t)p]) {
-        slot_of[(size_t)p] = (int)slots.size();
-        slots.push_back(p);
-      }
-    const int nL = (int)slots.size();
-    const size_t vat_need = (size_t)std::max(1, nL) * (size_t)std::max<int64_t>(1, e->n_sites);
-    if (e->vat_cap < vat_need) {
-      e->dfree(e->Vat, e->vat_cap * sizeof(double2));
-      e->Vat = (double2*)e->dalloc(vat_need * sizeof(double2));
-      e->vat_cap = vat_need;
-    }
-    // prepare units: on the line-kernel path two real (diagonal) densities
-    // share one complex prepare, their potentials being its real and
-    // imaginary parts (the kernel is real)
-    std::vector<std::array<int, 2>> units;
-    {
-      std::vector<int> reals;
-      const bool pk = e->pack && e->xline && e->shared();
-      for (int sl = 0; sl < nL; ++sl) {
-        int L = 0;
-        while ((L + 1) * (L + 2) / 2 <= slots[(size_t)sl]) ++L;
-        const bool diag = slots[(size_t)sl] == L * (L + 1) / 2 + L;
-        if (pk && diag) reals.push_back(sl);
-        else units.push_back({sl, -1});
-      }
-      for (size_t i = 0; i < reals.size(); i += 2)
-        units.push_back({reals[i], i + 1 < reals.size() ? reals[i + 1] : -1});
-    }
-    // overlap launch shape; its buffers are allocated here, before any
-    // capture of the device stage
-    const OvlCfg oc = overlap_cfg(e, nL, P);
-    ensure_overlap_buffers(e, oc);
-    const size_t n_out = oc.n_out;
-    std::vector<double>& ov = e->ov_host;  // kept between runs, like the plan buffer
-    ov.resize(n_out);
-    // with no sites every overlap is zero (the buffer holds the last run's)
-    if (!(nL > 0 && e->n_sites > 0)) {
-      std::fill(ov.begin(), ov.end(), 0.0);
-      HIPX(hipEventRecord(e->stage_ev[3], e->stream));
-    } else {
-      // prepares per launch: measured on the first run of this shape (the
-      // choices give bit-identical potentials), then the prepares and the
-      // overlaps as one replayed graph where it can be captured
-      const int nbt = choose_batch(e, units, slots);
-      device_stage(e, units, slots, nbt, oc);
-    }
-    HIPX(hipEventRecord(e->stage_ev[4], e->stream));
-    // the steps, on the host while the queued prepares and overlaps run (the
-    // copy of the overlap table to pageable memory below waits for them)
-    host::make_plan_into(quads, n, e->n_states, stale, e->plan_steps, &e->plan_in_cover);
-    const std::vector<host::PlanStep>& steps = e->plan_steps;
-    if (nL > 0 && e->n_sites > 0)
-      HIPX(hipMemcpyAsync(ov.data(), e->ovl, n_out * sizeof(double), hipMemcpyDeviceToHost, e->stream));
-    HIPX(hipStreamSynchronize(e->stream));
-    e->kcollect();
-    // main loop bookkeeping (coulombo.cpp:186-209)
-    auto value_of = [&](const host::PlanStep& s, int rc, double& vre, double& vim) {
-      const double* q = ov.data() + (((size_t)slot_of[(size_t)s.left.product]) * P + s.right.product) * 4;
-      const double Av = q[0], Bv = q[1], Cv = q[2], Dv = q[3];
-      if (rc) { vre = Av + Bv; vim = Dv - Cv; }
-      else { vre = Av - Bv; vim = Cv + Dv; }
-    };
-    if (!stale) {
-      // every value from its own step: host threads over id ranges
-      host::parallel_ranges(steps.size(), [&](size_t k0, size_t k1, size_t) {
-        for (size_t k = k0; k < k1; ++k) {
-          const auto& s = steps[k];
-          double vre, vim;
-          value_of(s, (s.left.conj != s.right.conj) ? 1 : 0, vre, vim);
-          out[2 * (size_t)s.id] = vre;
-          out[2 * (size_t)s.id + 1] = s.left.conj ? -vim : vim;
-        }
-      });
-    } else {
-      int lastR = -1, lastRc = 0;
-      double vre = 0.0, vim = 0.0;
-      for (const auto& s : steps) {
-        const int rc = (s.left.conj != s.right.conj) ? 1 : 0;
-        if (s.right.product != lastR || rc != lastRc) {
-          value_of(s, rc, vre, vim);
-          lastR = s.right.product;
-          lastRc = rc;
-        }
-        out[2 * (size_t)s.id] = vre;
-        out[2 * (size_t)s.id + 1] = s.left.conj ? -vim : vim;
-      }
-    }
-    float ms;
-    HIPX(hipEventElapsedTime(&ms, e->stage_ev[1], e->stage_ev[2])); e->stage_ms[1] = ms;
-    HIPX(hipEventElapsedTime(&ms, e->stage_ev[2], e->stage_ev[3])); e->stage_ms[2] = ms;
-    HIPX(hipEventElapsedTime(&ms, e->stage_ev[3], e->stage_ev[4])); e->stage_ms[3] = ms;
-    HIPX(hipEventElapsedTime(&ms, e->stage_ev[1], e->stage_ev[4])); e->stage_ms[4] = ms;
+    RunPlan rp;
+    run_begin(e, quads, n, flags, rp);
+    if (rp.device) device_stage(e, rp.units, rp.slots, rp.nbt, rp.oc);
+    else HIPX(hipEventRecord(e->stage_ev[3], e->stream));
+    run_finish(e, quads, n, rp, out);
+  });
+}
+
+int clat_compute_integrals_group(clat_engine* const* engines, int count, const int32_t* const* quads,
+                                 const int64_t* n, int flags, double* const* out) {
+  if (!engines || count <= 0 || !quads || !n || !out) return set_err(CLAT_EINVAL, "invalid arguments");
+  for (int i = 0; i < count; ++i) {
+    clat_engine* e = engines[i];
+    if (!e || (n[i] > 0 && (!quads[i] || !out[i])) || n[i] < 0) return set_err(CLAT_EINVAL, "invalid arguments");
+    for (int j = 0; j < i; ++j)
+      if (engines[j] == e) return set_err(CLAT_EINVAL, "an engine appears twice in the group");
+    if (!e->initialized) return set_err(CLAT_ESTATE, "calculator is not initialized");
+    if (e->n_states <= 0) return set_err(CLAT_ESTATE, "no states loaded");
+    if (e->device != engines[0]->device) return set_err(CLAT_EINVAL, "the engines of a group share one device");
+  }
+  return guarded([&] {
+    std::vector<RunPlan> rps((size_t)count);
+    for (int i = 0; i < count; ++i) run_begin(engines[i], quads[i], n[i], flags, rps[(size_t)i]);
+    group_stage(engines, count, rps);
+    for (int i = 0; i < count; ++i) run_finish(engines[i], quads[i], n[i], rps[(size_t)i], out[i]);
   });
 }
 

@@ -178,6 +178,23 @@ int clat_clear_states(clat_engine* e);
 int clat_compute_integrals(clat_engine* e, const int32_t* quads, int64_t n,
                            int flags, double* out);
 
+/* clat_compute_integrals for several systems at once (one engine each, any
+ * grid sizes, all on one device): quads[i], n[i] and out[i] belong to
+ * engines[i], flags to all.  No counterpart in the reference, which runs one
+ * system per process.  Each engine builds its pair densities and plan as in
+ * clat_compute_integrals; the device stages of all of them (prepares and
+ * overlaps) are then captured into one HIP graph on engines[0]'s stream and
+ * launched once per call, replayed while every member's plan and buffers stay
+ * the same (the graph belongs to engines[0]; a member that leads no group
+ * keeps its own graphs).  With kernel timing on in any member, or when the
+ * runtime refuses the capture, each engine launches its own device stage.
+ * Values are bit-identical to per-engine clat_compute_integrals calls.
+ * clat_last_stage_ms of each member reports the whole group device stage as
+ * [2] and 0 as [3].  An engine may appear once per group; the engines must
+ * not be in use by other threads during the call. */
+int clat_compute_integrals_group(clat_engine* const* engines, int count, const int32_t* const* quads,
+                                 const int64_t* n, int flags, double* const* out);
+
 /* potentials.cpp:140-158: V of the self product of `state` (1-based) at every
  * atom's cell (FunctionCollection::extractAtomCellValues,
  * FunctionCollection.cpp:122-131).  v: n_atoms complex values. */

@@ -111,3 +111,55 @@ def test_measured_batch_is_cached_and_bit_identical(monkeypatch):
         assert np.array_equal(v, va)
     for e in (a, b, c, d):
         e.close()
+
+
+def test_group_matches_per_engine_runs(monkeypatch):
+    """Three systems of different grids (32^3, 32x32x48, 64^3) in one group
+    call: their device stages go out as one HIP graph, captured on the first
+    call and replayed on later ones, and every member's values are
+    bit-identical to its own compute_integrals.  Between the group calls the
+    members do eager work (a potential, a solo run with another list) and the
+    last group changes one member's integral list, which re-captures.  A
+    group with a member off the line kernels (a 24x24x40 grid) launches each
+    stage directly, with the same values."""
+    print("case group graph", flush=True)
+    ws = [zb.workload(8, 8, 8, 2, 2, config=0, sigma_frac=0.2, n_orb=4),
+          zb.workload(8, 8, 12, 3, 2, config=0, sigma_frac=0.2, n_orb=4),
+          zb.workload(16, 16, 16, 2, 3, config=0, sigma_frac=0.2, n_orb=4)]
+    odd = zb.workload(6, 6, 10, 2, 2, config=0, sigma_frac=0.2, n_orb=4)
+    lists = [clat.pattern_enumerate("****", 4, 2), clat.pattern_enumerate("****", 5, 3),
+             clat.pattern_enumerate("****", 5, 3)]
+    other = clat.pattern_enumerate("eeee,hhhh", 5, 3)
+    for prec in (clat.PREC_FP64, clat.PREC_FP32X):
+        solo = [_engine(w, prec) for w in ws]
+        ref = [e.compute_integrals(q) for e, q in zip(solo, lists)]
+        ref_other = solo[2].compute_integrals(other)
+        ref_pot = solo[1].potential_at_atoms(2)
+        # every member runs on the line kernels (the only path with graphs)
+        assert all(e.tuning()["batch"] > 0 for e in solo), [e.tuning() for e in solo]
+        for e in solo:
+            e.close()
+        grp = [_engine(w, prec) for w in ws]
+        for k in range(4):
+            got = clat.compute_integrals_group(grp, lists)
+            for i in range(3):
+                assert np.array_equal(got[i], ref[i]), (prec, k, i)
+            assert np.array_equal(grp[1].potential_at_atoms(2), ref_pot)
+            if k == 1:
+                assert np.array_equal(grp[2].compute_integrals(other), ref_other)
+        got = clat.compute_integrals_group(grp, lists[:2] + [other])
+        assert np.array_equal(got[2], ref_other) and np.array_equal(got[0], ref[0])
+        t = grp[0].tuning()
+        assert t["graph_captures"] >= 2 and t["graph_replays"] >= 3, t
+        o_solo = _engine(odd, prec)
+        o_ref = o_solo.compute_integrals(lists[0])
+        o_solo.close()
+        o_eng = _engine(odd, prec)
+        for _ in range(2):
+            got = clat.compute_integrals_group([o_eng, grp[0]], lists[:1] + lists[:1])
+            assert np.array_equal(got[1], ref[0]) and np.array_equal(got[0], o_ref)
+        assert o_eng.tuning()["graph_captures"] == 0
+        with pytest.raises(clat.ClatError):
+            clat.compute_integrals_group([grp[0], grp[0]], lists[:2])
+        for e in grp + [o_eng]:
+            e.close()
