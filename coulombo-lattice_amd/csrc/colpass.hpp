@@ -347,8 +347,10 @@ __global__ __launch_bounds__((col_threads<C, R1>())) void k_colpass(const C2<TI>
         for (int k2 = 0; k2 < R2; ++k2) v[k2] = lds[(k2 * R1 + r) * C + c];
         rdft<R2, true, TC>(v);  // v[n2] = x_s[n1 + R1 n2] / W_2L^{n1 s}
         if (s == 1) rot_all<R2, true, TC>(v);
+        // the first sub-line initialises the sums (adding it to zero would
+        // cost R2 complex additions the compiler may not drop)
 #pragma unroll
-        for (int n2 = 0; n2 < R2; ++n2) acc[n2] = cadd<TC>(acc[n2], v[n2]);
+        for (int n2 = 0; n2 < R2; ++n2) acc[n2] = s == 0 ? v[n2] : cadd<TC>(acc[n2], v[n2]);
       }
     }
     if (s + 1 < NSUB) __syncthreads();

@@ -581,7 +581,7 @@ void launch_line(clat_engine* e, const void* in, void* out, long long bs, int ki
   const int R1 = e->cxf_s.R1, R2 = e->cxf_s.R2;
   int lines = 0;
   switch (R1 * 64 + R2) {
-#define CLAT_L2_N(ra, rb) case ra * 64 + rb: lines = clat_l2_lines_##ra##x##rb(sizeof(TC) == 8); break;
+#define CLAT_L2_N(ra, rb) case ra * 64 + rb: lines = clat_l2_lines_##ra##x##rb(combo); break;
     CLAT_P2_PAIRS(CLAT_L2_N)
 #undef CLAT_L2_N
     default: break;

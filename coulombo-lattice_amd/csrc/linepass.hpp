@@ -69,10 +69,14 @@ constexpr int line_threads() {
   return (m + 63) / 64 * 64;
 }
 // lines per block: about 256 threads (NSUB R2 items per line in one phase,
-// NSUB R1 in the others), half as many lines when the arithmetic is FP64
-template <typename TC, int R1, int R2, int NSUB>
+// NSUB R1 in the others), half as many lines when the arithmetic is FP64, and
+// half again for the FP64 inverse pass over complex64 input (its FP64 LDS tile
+// bounds the blocks per CU: with smaller blocks one more fits, and the input
+// loads alone do not keep the memory busy)
+template <typename TI, typename TC, int R1, int R2, int NSUB>
 constexpr int line_c() {
-  constexpr int q = (256 / (NSUB * (R1 > R2 ? R1 : R2))) / (sizeof(TC) == 8 ? 2 : 1);
+  constexpr int q = (256 / (NSUB * (R1 > R2 ? R1 : R2))) / (sizeof(TC) == 8 ? 2 : 1) /
+                    (sizeof(TC) == 8 && sizeof(TI) == 4 ? 2 : 1);
   return q >= 16 ? 16 : q >= 8 ? 8 : q >= 4 ? 4 : q >= 2 ? 2 : 1;
 }
 template <int R1>
@@ -264,7 +268,7 @@ struct LineLaunch {
 template <typename TI, typename TC, typename TO, int MODE, int R1, int R2>
 hipError_t line_go(const LineLaunch& L) {
   constexpr int NSUB = 2;
-  constexpr int C = line_c<TC, R1, R2, NSUB>();
+  constexpr int C = line_c<TI, TC, R1, R2, NSUB>();
   constexpr int nth = line_threads<C, R1, R2, NSUB>();
   constexpr size_t lds = line_lds<TC, R1, R2, NSUB, C>();
   static_assert(lds <= 65536, "line pass tile exceeds 64 KiB of LDS");
@@ -273,10 +277,18 @@ hipError_t line_go(const LineLaunch& L) {
   return hipGetLastError();
 }
 
-// lines per block of a pair, for the host's grid size
+// lines per block of a pair and type combination, for the host's grid size
 template <int R1, int R2>
-int line_lines(bool fp64_arith) {
-  return fp64_arith ? line_c<double, R1, R2, 2>() : line_c<float, R1, R2, 2>();
+int line_lines(int combo) {
+  using d = double;
+  using f = float;
+  switch (combo) {
+    case CB_DDD: case CB_DDF: return line_c<d, d, R1, R2, 2>();
+    case CB_DFF: return line_c<d, f, R1, R2, 2>();
+    case CB_FFD: return line_c<f, f, R1, R2, 2>();
+    case CB_FDD: return line_c<f, d, R1, R2, 2>();
+    default: return 0;
+  }
 }
 
 // (types, mode) pairs of the atom-path x passes; hipErrorInvalidValue otherwise
@@ -304,7 +316,7 @@ hipError_t line_launch(int combo, int mode, const LineLaunch& L) {
 
 #define CLAT_L2_DECL(a, b)                                                       \
   hipError_t clat_l2_##a##x##b(int combo, int mode, const LineLaunch& L);        \
-  int clat_l2_lines_##a##x##b(bool fp64_arith);
+  int clat_l2_lines_##a##x##b(int combo);
 CLAT_P2_PAIRS(CLAT_L2_DECL)
 
 }  // namespace clat

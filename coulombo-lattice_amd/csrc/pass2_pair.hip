@@ -22,5 +22,5 @@ hipError_t CLAT_C2_FN(R1V, R2V)(int combo, int mode, int nsub, const ColLaunch& 
 hipError_t CLAT_L2_FN(R1V, R2V)(int combo, int mode, const LineLaunch& L) {
   return line_launch<R1V, R2V>(combo, mode, L);
 }
-int CLAT_L2N_FN(R1V, R2V)(bool fp64_arith) { return line_lines<R1V, R2V>(fp64_arith); }
+int CLAT_L2N_FN(R1V, R2V)(int combo) { return line_lines<R1V, R2V>(combo); }
 }  // namespace clat
