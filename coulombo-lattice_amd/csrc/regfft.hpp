@@ -1,7 +1,14 @@
 // Register-resident DFTs of compile-time size R (1 <= R <= 20, factors 2,3,5,7)
-// for one work item.  Composite sizes use one Cooley-Tukey split R = P*M:
+// for one work item.  Composite sizes use one split R = P*M.  When P and M are
+// coprime (6, 10, 12, 14, 15, 20) it is the prime-factor (Good-Thomas) split,
+// which needs no twiddles: with n = (M n1 + P n2) mod R and
+// k = (M (M^-1 mod P) k1 + P (P^-1 mod M) k2) mod R,
+//   X[k] = sum_{n1<P} W_P^{n1 k1} sum_{n2<M} x[n] W_M^{n2 k2}
+// (both index maps are compile-time register renamings).  Otherwise (8 is a
+// base size; 9, 16, 18) one Cooley-Tukey split:
 //   X[M k1 + k2] = sum_{n1<P} W_P^{n1 k1} W_R^{n1 k2} sum_{n2<M} x[n1 + P n2] W_M^{n2 k2}
-// with the twiddles W_R^{e} folded to compile-time constants.
+// with the twiddles W_R^{e} folded to compile-time constants (the quarter
+// turns as swaps and sign changes).
 #pragma once
 
 #include "fft_device.hpp"
@@ -67,6 +74,14 @@ struct Split {  // first factor P (a base butterfly), rest M
   static constexpr int M = R / P;
 };
 
+constexpr int gcd_c(int a, int b) { return b == 0 ? a : gcd_c(b, a % b); }
+// a^-1 mod m (a and m coprime; m = 1 gives 0)
+constexpr int inv_mod_c(int a, int m) {
+  for (int x = 0; x < m; ++x)
+    if ((a * x) % m == 1 % m) return x;
+  return 0;
+}
+
 template <int R, bool INV, typename T>
 __device__ __forceinline__ void rdft(C2<T>* v);
 
@@ -86,6 +101,27 @@ __device__ __forceinline__ void rdft(C2<T>* v) {
   constexpr int P = Split<R>::P, M = Split<R>::M;
   if constexpr (R <= 5 || R == 7 || R == 8) {
     rdft_base<R, INV, T>(v);
+  } else if constexpr (gcd_c(P, M) == 1) {
+    constexpr int MI = inv_mod_c(M, P), PI = inv_mod_c(P, M);
+    C2<T> y[R];
+#pragma unroll
+    for (int n1 = 0; n1 < P; ++n1) {
+      C2<T> sub[M];
+#pragma unroll
+      for (int n2 = 0; n2 < M; ++n2) sub[n2] = v[(M * n1 + P * n2) % R];
+      rdft<M, INV, T>(sub);
+#pragma unroll
+      for (int k2 = 0; k2 < M; ++k2) y[n1 * M + k2] = sub[k2];
+    }
+#pragma unroll
+    for (int k2 = 0; k2 < M; ++k2) {
+      C2<T> t[P];
+#pragma unroll
+      for (int n1 = 0; n1 < P; ++n1) t[n1] = y[n1 * M + k2];
+      rdft_base<P, INV, T>(t);
+#pragma unroll
+      for (int k1 = 0; k1 < P; ++k1) v[(k1 * M * MI + k2 * P * PI) % R] = t[k1];
+    }
   } else {
     constexpr TwTable<R> tw{};
     C2<T> y[R];
@@ -98,12 +134,18 @@ __device__ __forceinline__ void rdft(C2<T>* v) {
 #pragma unroll
       for (int k2 = 0; k2 < M; ++k2) {
         const int e = (n1 * k2) % R;
+        const C2<T> a = sub[k2];
         if (e == 0) {
-          y[n1 * M + k2] = sub[k2];
+          y[n1 * M + k2] = a;
+        } else if (2 * e == R) {
+          y[n1 * M + k2] = mk<T>(-a.x, -a.y);
+        } else if (4 * e == R) {  // W^{R/4} = -i (forward), +i (inverse)
+          y[n1 * M + k2] = INV ? mk<T>(-a.y, a.x) : mk<T>(a.y, -a.x);
+        } else if (4 * e == 3 * R) {
+          y[n1 * M + k2] = INV ? mk<T>(a.y, -a.x) : mk<T>(-a.y, a.x);
         } else {
           const T c = (T)tw.c[e];
           const T s = INV ? (T)tw.s[e] : (T)(-tw.s[e]);
-          const C2<T> a = sub[k2];
           y[n1 * M + k2] = mk<T>(a.x * c - a.y * s, a.x * s + a.y * c);
         }
       }
