@@ -68,20 +68,32 @@ constexpr int line_threads() {
   constexpr int m = R2 > R1 ? C * NSUB * R2 : C * NSUB * R1;
   return (m + 63) / 64 * 64;
 }
-// lines per block: about 256 threads (NSUB R2 items per line in one phase,
-// NSUB R1 in the others), half as many lines when the arithmetic is FP64, and
-// half again for the FP64 inverse pass over complex64 input (its FP64 LDS tile
-// bounds the blocks per CU: with smaller blocks one more fits, and the input
-// loads alone do not keep the memory busy)
-template <typename TI, typename TC, int R1, int R2, int NSUB>
-constexpr int line_c() {
-  constexpr int q = (256 / (NSUB * (R1 > R2 ? R1 : R2))) / (sizeof(TC) == 8 ? 2 : 1) /
-                    (sizeof(TC) == 8 && sizeof(TI) == 4 ? 2 : 1);
-  return q >= 16 ? 16 : q >= 8 ? 8 : q >= 4 ? 4 : q >= 2 ? 2 : 1;
-}
 template <int R1>
 constexpr int line_r1p() {
   return (R1 > 1 && R1 % 2 == 0) ? R1 + 1 : R1;
+}
+// waves per CU that blocks of C lines of the FP64 inverse pass over
+// complex64 input can hold: by their LDS tile (160 KiB per CU) and by
+// registers (about 160 VGPRs: 12 waves)
+template <int R1, int R2, int NSUB>
+constexpr int line_fdd_waves(int C) {
+  const int lds = C * NSUB * R2 * line_r1p<R1>() * 16;
+  const int wpb = (C * NSUB * (R1 > R2 ? R1 : R2) + 63) / 64;
+  const int by_lds = (160 * 1024 / lds) * wpb;
+  return by_lds < 12 / wpb * wpb ? by_lds : 12 / wpb * wpb;
+}
+// lines per block: about 256 threads (NSUB R2 items per line in one phase,
+// NSUB R1 in the others), half as many lines when the arithmetic is FP64.
+// The FP64 inverse pass over complex64 input halves them again where that
+// fits more waves per CU (its FP64 LDS tile bounds the blocks, and its input
+// loads alone do not keep the memory busy: the 256-point lines)
+template <typename TI, typename TC, int R1, int R2, int NSUB>
+constexpr int line_c() {
+  constexpr int q0 = (256 / (NSUB * (R1 > R2 ? R1 : R2))) / (sizeof(TC) == 8 ? 2 : 1);
+  constexpr int q = q0 >= 16 ? 16 : q0 >= 8 ? 8 : q0 >= 4 ? 4 : q0 >= 2 ? 2 : 1;
+  if constexpr (sizeof(TC) == 8 && sizeof(TI) == 4 && q > 1)
+    return line_fdd_waves<R1, R2, NSUB>(q / 2) > line_fdd_waves<R1, R2, NSUB>(q) ? q / 2 : q;
+  return q;
 }
 template <typename TC, int R1, int R2, int NSUB, int C>
 constexpr size_t line_lds() {
