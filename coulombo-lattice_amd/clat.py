@@ -26,6 +26,8 @@ LIB_PATH = os.path.join(HERE, "libclat.so")
 PREC_FP64, PREC_FP32, PREC_FP32X = 0, 1, 2
 SCHED_SHARED, SCHED_ROUNDS = 0, 1
 FLAG_STALE_VALUES = 1
+# bits of clat_engine_kernels
+KERNEL_FLAGS = {"two_level": 1, "column": 2, "line": 4, "conv_tiled": 8, "b_strips": 16, "ad_strips": 32}
 
 _dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
 _ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
@@ -42,7 +44,8 @@ SYMBOLS = [
     "clat_compute_potential_at_atoms_noalloc",
     "clat_last_stage_ms", "clat_set_kernel_timing", "clat_set_kernel_timing_kinds", "clat_kernel_kind_count",
     "clat_kernel_kind_name",
-    "clat_kernel_stats", "clat_reset_kernel_stats", "clat_engine_info", "clat_set_tuning", "clat_tuning_info",
+    "clat_kernel_stats", "clat_reset_kernel_stats", "clat_engine_info", "clat_engine_kernels", "clat_set_tuning",
+    "clat_tuning_info",
 ]
 
 
@@ -98,6 +101,7 @@ def lib() -> ctypes.CDLL:
                                     ctypes.POINTER(ctypes.c_double)]
     L.clat_reset_kernel_stats.argtypes = [vp]
     L.clat_engine_info.argtypes = [vp, _ip, ctypes.POINTER(_i64)]
+    L.clat_engine_kernels.argtypes = [vp, ctypes.POINTER(ctypes.c_uint32)]
     L.clat_set_tuning.argtypes = [vp, ctypes.c_int]
     L.clat_tuning_info.argtypes = [vp, ctypes.POINTER(ctypes.c_int), _dp, ctypes.POINTER(_i64),
                                    ctypes.POINTER(_i64)]
@@ -246,6 +250,10 @@ class CoulombCalculator:
     def handle(self):
         return self._h.h
 
+    def kernels(self) -> set:
+        """Names of the kernel paths the passes run on (KERNEL_FLAGS)."""
+        return _kernels(self.handle)
+
     def initialize(self, interaction) -> None:
         if isinstance(interaction, InteractionTable):
             interaction._apply(self.handle, self.dims)
@@ -381,8 +389,18 @@ class Engine:
         _check(lib().clat_engine_info(self.handle, dim, ctypes.byref(nb)))
         return tuple(int(v) for v in dim), nb.value
 
+    def kernels(self) -> set:
+        """Names of the kernel paths the engine's passes run on (KERNEL_FLAGS)."""
+        return _kernels(self.handle)
+
     def close(self):
         self._h.close()
+
+
+def _kernels(handle) -> set:
+    f = ctypes.c_uint32()
+    _check(lib().clat_engine_kernels(handle, ctypes.byref(f)))
+    return {k for k, bit in KERNEL_FLAGS.items() if f.value & bit}
 
 
 def compute_integrals_group(engines, quad_lists, stale_values: bool = False) -> list:

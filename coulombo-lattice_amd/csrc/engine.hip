@@ -122,9 +122,19 @@ bool split2(int L, int& R1, int& R2) {
   return true;
 }
 
-// engine axis length for a user length: the smallest two-level size, else
-// the next 2*3*5*7-smooth size (generic kernels)
+bool col_pair_compiled(int R1, int R2);
+
+// engine axis length for a user length: the smallest size whose two-level
+// split has compiled kernels (every 2*3*5-smooth length from 32 to 400 that
+// splits has them; the open-boundary convolution does not depend on the
+// padding), else the smallest two-level size, else the next 2*3*5*7-smooth
+// size (generic kernels)
 int engine_len(int n) {
+  if (n >= 32)
+    for (int m = n; m <= 400; ++m) {
+      int a, b;
+      if (split2(m, a, b) && col_pair_compiled(a, b)) return m;
+    }
   for (int m = n; m <= 400; ++m) {
     int a, b;
     if (split2(m, a, b)) return m;
@@ -2371,6 +2381,19 @@ int clat_engine_info(clat_engine* e, int dim[3], int64_t* device_bytes) {
   if (!e) return set_err(CLAT_EINVAL, "invalid arguments");
   if (dim) { dim[0] = e->Xi; dim[1] = e->Yi; dim[2] = e->Zi; }
   if (device_bytes) *device_bytes = e->device_bytes;
+  return CLAT_OK;
+}
+
+int clat_engine_kernels(clat_engine* e, uint32_t* flags) {
+  if (!e || !flags) return set_err(CLAT_EINVAL, "invalid arguments");
+  uint32_t f = 0;
+  if (e->fast) f |= CLAT_KERNELS_TWO_LEVEL;
+  if (e->fast && e->colk && col_pair_compiled(e->cyf.R1, e->cyf.R2)) f |= CLAT_KERNELS_COLUMN;
+  if (e->xline) f |= CLAT_KERNELS_LINE;
+  if (e->conv_tiled) f |= CLAT_KERNELS_CONV_TILED;
+  if (e->btile) f |= CLAT_KERNELS_B_STRIPS;
+  if (e->atile) f |= CLAT_KERNELS_AD_STRIPS;
+  *flags = f;
   return CLAT_OK;
 }
 

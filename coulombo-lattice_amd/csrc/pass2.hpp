@@ -387,7 +387,12 @@ hipError_t p2_launch(int combo, int mode, bool line, const P2Launch& L) {
 }
 
 // compile-time radix pairs with their own code objects (pass2_pair.hip)
-#define CLAT_P2_PAIRS(X) X(16, 16) X(16, 8) X(8, 4) X(8, 8) X(9, 5) X(12, 8) X(12, 10) X(16, 10) X(16, 12) X(20, 20)
+#define CLAT_P2_PAIRS(X)                                                                 \
+  X(16, 16) X(16, 8) X(8, 4) X(8, 8) X(9, 5) X(12, 8) X(12, 10) X(16, 10) X(16, 12) X(20, 20) \
+  X(6, 6) X(8, 5) X(8, 6) X(10, 5) X(9, 6) X(10, 6) X(9, 8) X(15, 5) \
+  X(10, 8) X(9, 9) X(10, 9) X(10, 10) X(12, 9) X(15, 9) X(12, 12) X(15, 10) \
+  X(18, 9) X(15, 12) X(20, 10) X(18, 12) X(15, 15) X(16, 15) X(18, 15) X(18, 16) \
+  X(20, 15) X(20, 16) X(18, 18) X(20, 18)
 #define CLAT_P2_DECL(a, b) hipError_t clat_p2_##a##x##b(int combo, int mode, bool line, const P2Launch& L);
 CLAT_P2_PAIRS(CLAT_P2_DECL)
 

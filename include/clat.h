@@ -246,6 +246,20 @@ int clat_reset_kernel_stats(clat_engine* e);
  * bytes it currently holds. */
 int clat_engine_info(clat_engine* e, int dim[3], int64_t* device_bytes);
 
+/* Which kernels the engine's passes run on (bits of *flags): the two-level
+ * register FFT kernels (else the LDS Stockham kernels), compile-time column
+ * kernels for the y passes, the x line kernels of the atom path (occupied
+ * lines only), the tiled conv kernel table, and the column-strip layouts of
+ * the intermediate planes.  An introspection hook with no counterpart in the
+ * reference: every combination gives the same results. */
+#define CLAT_KERNELS_TWO_LEVEL 1u
+#define CLAT_KERNELS_COLUMN 2u
+#define CLAT_KERNELS_LINE 4u
+#define CLAT_KERNELS_CONV_TILED 8u
+#define CLAT_KERNELS_B_STRIPS 16u
+#define CLAT_KERNELS_AD_STRIPS 32u
+int clat_engine_kernels(clat_engine* e, uint32_t* flags);
+
 /* ---- planning ---------------------------------------------------------- */
 
 /* The measureFFT choice of CoulombCalculator(const Dimension&, bool

@@ -93,6 +93,68 @@ def test_column_kernels_match_oracle(prec, sched, dims):
     assert err < TOL[prec] * 0.1, err
 
 
+# every 2*3*5-smooth length from 36 to 360 whose two-level split was added to
+# the compiled radix pairs beyond the ten above, twice per grid (y and z)
+ADDED_YZ = [(2, 36, 40), (3, 48, 50), (2, 54, 60), (3, 72, 75), (2, 80, 81), (3, 90, 100), (2, 108, 135),
+            (3, 144, 150), (2, 162, 180), (2, 200, 216), (2, 225, 240), (2, 270, 288), (2, 300, 320),
+            (2, 324, 360)]
+ADDED_X = [36, 40, 48, 50, 54, 60, 72, 75, 80, 81, 90, 100, 108, 135, 144, 150, 162, 180, 200, 216, 225, 240,
+           270, 288, 300, 320, 324, 360]
+SHARED = [(clat.PREC_FP64, clat.SCHED_SHARED), (clat.PREC_FP32X, clat.SCHED_SHARED),
+          (clat.PREC_FP32, clat.SCHED_SHARED)]
+
+
+@pytest.mark.parametrize("prec,sched", SHARED, ids=["fp64", "fp32x", "fp32"])
+@pytest.mark.parametrize("dims", ADDED_YZ)
+def test_added_column_pairs_match_oracle(prec, sched, dims):
+    """y and z lengths on the radix pairs of the other 2*3*5-smooth lengths
+    (6x6 ... 20x18: 36 to 360), against the 8-round oracle."""
+    X, Y, Z = dims
+    print("case added column", dims, prec, flush=True)
+    rng = np.random.default_rng(X * 1000003 + Y * 1009 + Z)
+    F = rng.standard_normal((Z, Y, X)) + 1j * rng.standard_normal((Z, Y, X))
+    step = (0.8, 1.2, 1.0)
+    Vo = O.prepare(F, O.ghat(O.gtable(X, Y, Z, step, 0.5, 2.0)))
+    cc = clat.CoulombCalculator(dims, prec, sched)
+    # no padding (the pairs are compiled) and the compile-time column kernels
+    assert {"two_level", "column", "conv_tiled"} <= cc.kernels(), cc.kernels()
+    cc.initialize(clat.InteractionSimple(step, 0.5, 2.0))
+    cc.input[...] = F
+    cc.prepare()
+    err = np.max(np.abs(cc.potential() - Vo)) / np.max(np.abs(Vo))
+    assert err < TOL[prec] * 0.1, err
+
+
+@pytest.mark.parametrize("prec", [clat.PREC_FP64, clat.PREC_FP32X], ids=["fp64", "fp32x"])
+@pytest.mark.parametrize("X", ADDED_X)
+def test_added_line_pairs_match_oracle(prec, X):
+    """Atom path (x line kernels over occupied lines, y passes on the
+    36-point pair skipping empty rows) on x lengths of the added radix
+    pairs."""
+    Y, Z = 36, 5
+    print("case added lines", X, prec, flush=True)
+    rng = np.random.default_rng(X * 7919 + 17)
+    lines = rng.choice(Y * Z, size=(Y * Z * 3) // 5, replace=False)
+    cells = []
+    for l in lines:
+        xs = rng.choice(X, size=int(rng.integers(1, max(2, X // 3))), replace=False)
+        cells += [int(l) * X + int(x) for x in xs]
+    cells = np.array(cells, dtype=np.int32)
+    psi = rng.standard_normal((len(cells), 2)) + 1j * rng.standard_normal((len(cells), 2))
+    step = (1.0, 1.1, 0.9)
+    eng = clat.Engine((X, Y, Z), prec, clat.SCHED_SHARED)
+    assert eng.info()[0] == (X, Y, Z) and "line" in eng.kernels(), (eng.info(), eng.kernels())
+    eng.initialize(clat.InteractionSimple(step, 0.3, 2.5))
+    eng.set_atoms(cells, 2)
+    eng.add_state(psi)
+    v = eng.potential_at_atoms(1)
+    eng.close()
+    F = np.zeros(X * Y * Z, complex)
+    F[cells] = np.sum(np.abs(psi) ** 2, axis=1)
+    Vo = O.prepare(F.reshape(Z, Y, X), O.ghat(O.gtable(X, Y, Z, step, 0.3, 2.5))).reshape(-1)[cells]
+    assert np.max(np.abs(v - Vo)) / np.max(np.abs(Vo)) < TOL[prec]
+
+
 def _rel(a, b):
     return float(np.max(rel_err(a, b)))
 
