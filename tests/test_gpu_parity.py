@@ -360,12 +360,13 @@ def test_potential_at_atoms(prec):
 
 
 @pytest.mark.parametrize("prec", [clat.PREC_FP64, clat.PREC_FP32, clat.PREC_FP32X], ids=["fp64", "fp32", "fp32x"])
-@pytest.mark.parametrize("dims", [(32, 6, 5), (45, 4, 3), (64, 3, 7), (96, 5, 2), (120, 3, 3), (128, 4, 4),
-                                  (160, 2, 3), (192, 3, 2), (256, 5, 3), (400, 2, 2)])
+@pytest.mark.parametrize("dims", [(32, 32, 5), (45, 36, 3), (64, 32, 7), (96, 36, 2), (120, 32, 3),
+                                  (128, 36, 4), (160, 32, 3), (192, 36, 2), (256, 32, 3), (400, 32, 2)])
 def test_site_lines_match_oracle(prec, dims):
-    """Atom path on x lengths of every compile-time radix pair: random sites
-    on a random subset of x-lines (empty lines skipped by the x passes and
-    never touched by the y passes), several atoms per cell (the last one
+    """Atom path on x lengths of the first ten compile-time radix pairs (y on
+    a compiled pair too, so the x passes run on the line kernels): random
+    sites on a random subset of x-lines (empty lines skipped by the x passes
+    and never touched by the y passes), several atoms per cell (the last one
     holds the cell), an occupied-line count that leaves a ragged last block."""
     X, Y, Z = dims
     print("case site_lines", dims, prec, flush=True)
@@ -381,6 +382,7 @@ def test_site_lines_match_oracle(prec, dims):
     psi = rng.standard_normal((len(cells), n_orb)) + 1j * rng.standard_normal((len(cells), n_orb))
     step = (1.0, 1.1, 0.9)
     eng = clat.Engine((X, Y, Z), prec, clat.SCHED_SHARED)
+    assert "line" in eng.kernels(), eng.kernels()
     eng.initialize(clat.InteractionSimple(step, 0.3, 2.5))
     eng.set_atoms(cells, n_orb)
     eng.add_state(psi)
