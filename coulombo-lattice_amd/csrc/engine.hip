@@ -308,6 +308,7 @@ struct clat_engine {
   std::vector<host::PlanStep> plan_steps;
   std::vector<char> plan_in_cover, plan_leads;
   std::vector<double> ov_host;
+  bool ovl_sym = true;  // symmetric MFMA overlap table (CLAT_OVL_SYM=0: every tile)
   int ovl_mode = 0;  // overlap kernel: 0 by size, 1 vector FMA, 2 FP64 MFMA, 3 vector FMA in 48 x 48 tiles,
                     // 4 the same with every site sum Kahan-compensated
   bool table_pending = false;           // stage_ms[0] of the last table build not read yet
@@ -1219,6 +1220,7 @@ using Units = std::vector<std::array<int, 2>>;  // (slot, second real slot or -1
 // otherwise (or Kahan-compensated 48 x 48 when asked for)
 struct OvlCfg {
   bool mfma = false, kahan = false, t36 = false;
+  bool sym = false;  // MFMA tiles on and above the diagonal only (every product a left slot)
   int OV_T = 48, tl = 0, tr = 0, groups = 1;
   int64_t n_chunks = 1, chunk = 1, n_slices = 1;
   size_t n_out = 0;
@@ -1234,11 +1236,13 @@ OvlCfg overlap_cfg(const clat_engine* e, int nL, int P) {
   c.groups = c.t36 ? 3 : 1;  // site groups per block, a partial slice each
   c.tl = (nL + c.OV_T - 1) / c.OV_T;
   c.tr = (P + c.OV_T - 1) / c.OV_T;
+  c.sym = c.mfma && nL == P && e->ovl_sym;
+  const int tiles = c.sym ? c.tl * (c.tl + 1) / 2 : c.tl * c.tr;
   // blocks: about 2048 for the 48 x 48 and MFMA tiles; for the 36 x 36
   // tiles (246 VGPRs: two blocks per CU) one full round, two per CU
   // (0.33 ms against 0.37 at 2.7 blocks per CU and 0.45 at 8, 1M atoms)
   const int want_blocks = c.t36 ? 2 * e->n_cu : 2048;
-  int64_t n_chunks = std::max<int64_t>(1, want_blocks / std::max(1, c.tl * c.tr));
+  int64_t n_chunks = std::max<int64_t>(1, want_blocks / std::max(1, tiles));
   n_chunks = std::min<int64_t>(n_chunks, (e->n_sites + ovs - 1) / ovs);
   n_chunks = std::max<int64_t>(1, n_chunks);
   int64_t chunk = (e->n_sites + n_chunks - 1) / n_chunks;
@@ -1268,9 +1272,12 @@ void ensure_overlap_buffers(clat_engine* e, const OvlCfg& c) {
 void launch_overlaps(clat_engine* e, int nL, int P, const OvlCfg& c) {
   e->kbegin(K_OVERLAP);
   const dim3 og(c.tl, c.tr, (unsigned)c.n_chunks);
-  if (c.mfma)
+  if (c.mfma && c.sym)
+    hipLaunchKernelGGL(k_overlap_partial_mfma, dim3((unsigned)(c.tl * (c.tl + 1) / 2), 1, (unsigned)c.n_chunks),
+                       dim3(256), 0, e->stream, e->Vat, e->rho, nL, P, e->n_sites, c.chunk, e->part, c.tl);
+  else if (c.mfma)
     hipLaunchKernelGGL(k_overlap_partial_mfma, og, dim3(256), 0, e->stream, e->Vat, e->rho, nL, P, e->n_sites,
-                       c.chunk, e->part);
+                       c.chunk, e->part, 0);
   else if (c.t36)
     hipLaunchKernelGGL((k_overlap_partial<9, 4, 3, 24>), og, dim3(243), 0, e->stream, e->Vat, e->rho, nL, P,
                        e->n_sites, c.chunk, e->part);
@@ -1284,7 +1291,8 @@ void launch_overlaps(clat_engine* e, int nL, int P, const OvlCfg& c) {
   hipLaunchKernelGGL(k_overlap_reduce, dim3((unsigned)((c.n_out + 63) / 64)), dim3(1024), 0, e->stream, e->part,
                      c.n_slices, (int64_t)c.n_out, e->ovl);
   LAUNCH_CHECK();
-  e->kend(K_OVERLAP, (double)(c.tl * c.tr) * (double)(2 * c.OV_T) * (double)e->n_sites * 16.0);
+  e->kend(K_OVERLAP, (double)(c.sym ? c.tl * (c.tl + 1) / 2 : c.tl * c.tr) * (double)(2 * c.OV_T) *
+                        (double)e->n_sites * 16.0);
 }
 
 // every prepare of the run, nbt per pass launch: pair density of slot
@@ -1387,7 +1395,7 @@ std::vector<int64_t> stage_key(const clat_engine* e, const Units& units, const s
                                const OvlCfg& c) {
   std::vector<int64_t> k = {nbt, (int64_t)units.size(), (int64_t)slots.size(), (int64_t)(uintptr_t)e->rho,
                             (int64_t)(uintptr_t)e->Vat, (int64_t)(uintptr_t)e->part, (int64_t)(uintptr_t)e->ovl,
-                            e->n_sites, e->atoms_gen, c.mfma, c.kahan, c.t36, c.tl, c.tr, c.n_chunks, c.chunk,
+                            e->n_sites, e->atoms_gen, c.mfma, c.kahan, c.t36, c.sym, c.tl, c.tr, c.n_chunks, c.chunk,
                             (int64_t)c.n_out};
   for (const auto& u : units) {
     k.push_back(u[0]);
@@ -1559,8 +1567,17 @@ void run_finish(clat_engine* e, const int32_t* quads, int64_t n, const RunPlan& 
   e->kcollect();
   const int P = rp.P;
   auto value_of = [&](const host::PlanStep& s, int rc, double& vre, double& vim) {
-    const double* q = ov.data() + (((size_t)rp.slot_of[(size_t)s.left.product]) * P + s.right.product) * 4;
-    const double Av = q[0], Bv = q[1], Cv = q[2], Dv = q[3];
+    const int sl = rp.slot_of[(size_t)s.left.product], pr = s.right.product;
+    double Av, Bv, Cv, Dv;
+    if (rp.oc.sym && sl / OVM_T > pr / OVM_T) {
+      // below the diagonal: the mirror entry (slot pr, product sl), its
+      // mixed sums exchanged (every product is a slot: slot = product)
+      const double* q = ov.data() + ((size_t)pr * P + sl) * 4;
+      Av = q[0]; Bv = q[1]; Cv = q[3]; Dv = q[2];
+    } else {
+      const double* q = ov.data() + ((size_t)sl * P + pr) * 4;
+      Av = q[0]; Bv = q[1]; Cv = q[2]; Dv = q[3];
+    }
     if (rc) { vre = Av + Bv; vim = Dv - Cv; }
     else { vre = Av - Bv; vim = Cv + Dv; }
   };
@@ -1959,6 +1976,7 @@ int clat_create(int X, int Y, int Z, int precision, int schedule, clat_engine** 
     }
     if (const char* g = std::getenv("CLAT_TUNE")) e->tune = e->tune && std::atoi(g) != 0;
     if (const char* g = std::getenv("CLAT_GRAPH")) e->use_graph = std::atoi(g) != 0;
+    if (const char* g = std::getenv("CLAT_OVL_SYM")) e->ovl_sym = std::atoi(g) != 0;
     if (const char* g = std::getenv("CLAT_PACK")) e->pack = std::atoi(g) != 0;
     if (const char* g = std::getenv("CLAT_OVL"))
       e->ovl_mode = std::strcmp(g, "valu") == 0 ? 1 : std::strcmp(g, "mfma") == 0 ? 2

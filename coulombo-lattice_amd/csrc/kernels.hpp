@@ -738,6 +738,12 @@ __global__ __launch_bounds__(TX * TX * G) void k_overlap_partial(const double2* 
 // i + 16k (slot i, site k), B[k][j] in lane j + 16k (site k, product j);
 // result D[i][j] in lane j + 16 (i % 4), element i / 4.
 // Same chunking and partial layout as k_overlap_partial.
+// tri > 0: every product is a left slot (n_left == n_right, slot index =
+// product index) and the table is symmetric: sum V_a rho_b = sum rho_a V_b
+// (the kernel G is real and even), so only the tri (tri + 1) / 2 tiles on or
+// above the diagonal are formed, blockIdx.x walking them row by row; the
+// host reads an entry below the diagonal from its mirror with C and D
+// exchanged.
 constexpr int OVM_T = 64;   // tile edge (slots and products)
 constexpr int OVM_S = 16;   // sites staged per step
 constexpr int OVM_MIN = 96; // both sides at least this many: the MFMA tiles are used
@@ -745,13 +751,24 @@ constexpr int OVM_MIN = 96; // both sides at least this many: the MFMA tiles are
 __global__ __launch_bounds__(256) void k_overlap_partial_mfma(const double2* __restrict__ Vat,
                                                               const double2* __restrict__ rho,
                                                               int n_left, int n_right, int64_t n_sites,
-                                                              int64_t chunk, double* __restrict__ part) {
+                                                              int64_t chunk, double* __restrict__ part, int tri) {
   typedef double d4 __attribute__((ext_vector_type(4)));
   constexpr int P = OVM_S + 1;  // LDS pitch in double2
   __shared__ double2 sv[OVM_T * P];
   __shared__ double2 sr[OVM_T * P];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  const int l0 = blockIdx.x * OVM_T, r0 = blockIdx.y * OVM_T;
+  int tl = (int)blockIdx.x, tr = (int)blockIdx.y;
+  if (tri > 0) {
+    // upper-triangle tile number -> (row tl, column tr >= tl)
+    int t = (int)blockIdx.x;
+    tl = 0;
+    while (t >= tri - tl) {
+      t -= tri - tl;
+      ++tl;
+    }
+    tr = tl + t;
+  }
+  const int l0 = tl * OVM_T, r0 = tr * OVM_T;
   const int64_t s0 = (int64_t)blockIdx.z * chunk;
   const int64_t s1 = min(n_sites, s0 + chunk);
   d4 acc[4][4];
