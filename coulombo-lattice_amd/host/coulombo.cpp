@@ -38,8 +38,8 @@ const char* kUsage =
     "  --tf-lattice=VALUE     lattice constant (A) for Thomas-Fermi-Resta model,\n"
     "                           default: model not applied\n"
     "  --precision=P          fp64 (default); fp32x: complex64 passes, FP64 last pass and sums\n"
-    "                           (each integral within 1e-5 of itself vs fp64 down to 1e-6 of the\n"
-    "                           largest, within 1e-11 of the largest below); fp32: complex64\n"
+    "                           (each integral within 1e-5 of itself vs fp64, or within 3e-10 of\n"
+    "                           the largest integral if that is larger); fp32: complex64\n"
     "                           throughout (within 1e-5 of max(|I|, 1e-3 of the largest))\n"
     "  --schedule=S           shared (default): the 8 rounds share x/y passes; rounds: 8 full rounds\n"
     "\nFLAGS:\n"

@@ -44,13 +44,16 @@ extern "C" {
                              but the last; the last (inverse x) pass computes
                              in FP64 and writes the potential in FP64.  On the
                              1M-atom, 8-state workload every integral is
-                             within 1e-5 of itself against FP64 (5.6e-6).  In
-                             general: within 1e-5 of itself for integrals of
-                             at least 1e-6 of the run's largest, within 1e-11
-                             of the largest below that -- rounding a pair
-                             density to complex64 at all moves integrals that
-                             cancel to ~1e-8 of the largest by more than 1e-5
-                             of themselves                                  */
+                             within 1e-5 of itself against FP64 (5.6e-6); on
+                             the batched and six-box workloads every integral
+                             of at least 1e-6 of the run's largest is, and
+                             the rest are within 1e-11 of the largest.  In
+                             general (small grids included): within 1e-5 of
+                             itself or 3e-10 of the run's largest, whichever
+                             is larger -- rounding a pair density to
+                             complex64 at all moves integrals that cancel to
+                             ~1e-8 of the largest by more than 1e-5 of
+                             themselves                                     */
 
 /* Schedule of the eight half-shift rounds (CoulombCalculator.cpp:105-163). */
 #define CLAT_SCHED_SHARED 0  /* rounds share their x and y passes: the eight
