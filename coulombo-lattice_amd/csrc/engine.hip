@@ -307,7 +307,8 @@ struct clat_engine {
   // host buffers of a run, kept with their pages between runs
   std::vector<host::PlanStep> plan_steps;
   std::vector<char> plan_in_cover, plan_leads;
-  std::vector<double> ov_host;
+  double* ov_host = nullptr;            // overlap table on the host, pinned (a faster copy back)
+  size_t ov_host_cap = 0;
   bool ovl_sym = true;  // symmetric MFMA overlap table (CLAT_OVL_SYM=0: every tile)
   int ovl_mode = 0;  // overlap kernel: 0 by size, 1 vector FMA, 2 FP64 MFMA, 3 vector FMA in 48 x 48 tiles,
                     // 4 the same with every site sum Kahan-compensated
@@ -1543,10 +1544,16 @@ void run_begin(clat_engine* e, const int32_t* quads, int64_t n, int flags, RunPl
   // capture of the device stage
   rp.oc = overlap_cfg(e, rp.nL, rp.P);
   ensure_overlap_buffers(e, rp.oc);
-  e->ov_host.resize(rp.oc.n_out);  // kept between runs, like the plan buffer
+  if (e->ov_host_cap < rp.oc.n_out) {  // kept between runs, like the plan buffer
+    if (e->ov_host) (void)hipHostFree(e->ov_host);
+    e->ov_host = nullptr;
+    e->ov_host_cap = 0;
+    HIPX(hipHostMalloc((void**)&e->ov_host, rp.oc.n_out * sizeof(double), hipHostMallocDefault));
+    e->ov_host_cap = rp.oc.n_out;
+  }
   rp.device = rp.nL > 0 && e->n_sites > 0;
   // with no sites every overlap is zero (the buffer holds the last run's)
-  if (!rp.device) std::fill(e->ov_host.begin(), e->ov_host.end(), 0.0);
+  if (!rp.device) std::fill(e->ov_host, e->ov_host + rp.oc.n_out, 0.0);
   // prepares per launch: measured on the first run of this shape (the
   // choices give bit-identical potentials)
   else rp.nbt = choose_batch(e, rp.units, rp.slots);
@@ -1556,13 +1563,13 @@ void run_begin(clat_engine* e, const int32_t* quads, int64_t n, int flags, RunPl
 // the host, the main loop's bookkeeping (coulombo.cpp:186-209), stage times.
 void run_finish(clat_engine* e, const int32_t* quads, int64_t n, const RunPlan& rp, double* out) {
   HIPX(hipEventRecord(e->stage_ev[4], e->stream));
-  // the copy of the overlap table to pageable memory below waits for the
-  // queued prepares and overlaps
+  // the copy of the overlap table below waits for the queued prepares and
+  // overlaps
   host::make_plan_into(quads, n, e->n_states, rp.stale, e->plan_steps, &e->plan_in_cover);
   const std::vector<host::PlanStep>& steps = e->plan_steps;
-  std::vector<double>& ov = e->ov_host;
+  const double* ov = e->ov_host;
   if (rp.device)
-    HIPX(hipMemcpyAsync(ov.data(), e->ovl, rp.oc.n_out * sizeof(double), hipMemcpyDeviceToHost, e->stream));
+    HIPX(hipMemcpyAsync(e->ov_host, e->ovl, rp.oc.n_out * sizeof(double), hipMemcpyDeviceToHost, e->stream));
   HIPX(hipStreamSynchronize(e->stream));
   e->kcollect();
   const int P = rp.P;
@@ -1572,10 +1579,10 @@ void run_finish(clat_engine* e, const int32_t* quads, int64_t n, const RunPlan& 
     if (rp.oc.sym && sl / OVM_T > pr / OVM_T) {
       // below the diagonal: the mirror entry (slot pr, product sl), its
       // mixed sums exchanged (every product is a slot: slot = product)
-      const double* q = ov.data() + ((size_t)pr * P + sl) * 4;
+      const double* q = ov + ((size_t)pr * P + sl) * 4;
       Av = q[0]; Bv = q[1]; Cv = q[3]; Dv = q[2];
     } else {
-      const double* q = ov.data() + ((size_t)sl * P + pr) * 4;
+      const double* q = ov + ((size_t)sl * P + pr) * 4;
       Av = q[0]; Bv = q[1]; Cv = q[2]; Dv = q[3];
     }
     if (rc) { vre = Av + Bv; vim = Dv - Cv; }
@@ -2023,6 +2030,7 @@ int clat_destroy(clat_engine* e) {
     if (g) (void)hipGraphExecDestroy(g);
   if (e->ggroup) (void)hipGraphExecDestroy(e->ggroup);
   if (e->join_ev) (void)hipEventDestroy(e->join_ev);
+  if (e->ov_host) (void)hipHostFree(e->ov_host);
   if (e->stream) (void)hipStreamDestroy(e->stream);
   delete e;
   return CLAT_OK;
