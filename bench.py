@@ -272,6 +272,7 @@ def batched_form(args, clat, zb, np):
                        % (len(w["cells"]), X, Y, Z),
            "integrals": int(len(quads)), "steps": steps, "ms_per_step": ms,
            "integrals_per_s": len(quads) / (r["total"] / steps),
+           "graph_replay_ms_per_step": 1e3 * r["total_graph"] / steps,
            "fp64_ms_per_step": 1e3 * r64["total"],
            "stage_ms": dict(zip(("table", "pair_densities", "prepares", "overlaps", "call"), r["stage"])),
            "roofline": roofline_of(r, 8.0 * n_left * P * len(w["cells"])),
