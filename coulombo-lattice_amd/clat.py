@@ -35,7 +35,8 @@ _i64 = ctypes.c_int64
 
 # every symbol include/clat.h declares, for the export check
 SYMBOLS = [
-    "clat_last_error", "clat_version", "clat_round_up", "clat_lattice_grid", "clat_lattice_cells",
+    "clat_last_error", "clat_version", "clat_round_up", "clat_engine_length", "clat_lattice_grid",
+    "clat_lattice_cells",
     "clat_pattern_enumerate", "clat_vertex_cover", "clat_plan", "clat_create", "clat_destroy",
     "clat_set_interaction_simple", "clat_set_interaction_thomas_fermi", "clat_set_interaction_table",
     "clat_set_input", "clat_prepare", "clat_calculate", "clat_get_potential", "clat_set_atoms",
@@ -69,6 +70,7 @@ def lib() -> ctypes.CDLL:
     L.clat_last_error.restype = ctypes.c_char_p
     L.clat_version.restype = ctypes.c_char_p
     L.clat_round_up.argtypes = [ctypes.c_int, ctypes.c_int, _ip, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]
+    L.clat_engine_length.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_int)]
     L.clat_lattice_grid.argtypes = [_dp, _i64, _ip, _ip, _dp, _dp]
     L.clat_lattice_cells.argtypes = [_dp, _i64, _ip, _dp, _dp, _ip]
     L.clat_pattern_enumerate.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.c_int, vp, _i64,
@@ -123,6 +125,13 @@ def _c(a) -> np.ndarray:
 def round_up(n: int, c: int = 1, primes=(2, 3, 5)) -> int:
     out = ctypes.c_int()
     _check(lib().clat_round_up(n, c, np.asarray(primes, np.int32), len(primes), ctypes.byref(out)))
+    return out.value
+
+
+def engine_length(n: int) -> int:
+    """The axis length an engine computes on for a user length n."""
+    out = ctypes.c_int()
+    _check(lib().clat_engine_length(n, ctypes.byref(out)))
     return out.value
 
 

@@ -1741,6 +1741,13 @@ int clat_round_up(int n, int c, const int* primes, int n_primes, int* out) {
   });
 }
 
+int clat_engine_length(int n, int* out) {
+  return guarded([&] {
+    if (n <= 0 || !out) throw host::Error(CLAT_EINVAL, "invalid arguments");
+    *out = engine_len(n);
+  });
+}
+
 int clat_lattice_grid(const double* xyz, int64_t n, int dim_raw[3], int dim_pad[3], double origin[3],
                       double step[3]) {
   return guarded([&] {

@@ -81,6 +81,13 @@ const char* clat_version(void);
  * c * p1^a * p2^b * ...   Replaces src/Round.hpp:20-27,38-49. */
 int clat_round_up(int n, int c, const int* primes, int n_primes, int* out);
 
+/* The axis length an engine computes on for a user length n: n itself when
+ * its two-level split has compile-time kernels, else the smallest such
+ * length above n (n >= 32; exact: the open-boundary convolution does not
+ * depend on the padding), else the smallest two-level length, else the next
+ * 2*3*5*7-smooth length.  No counterpart in the reference. */
+int clat_engine_length(int n, int* out);
+
 /* Lattice detection of src/Broadcaster.cpp:24-43,53-56 (computeGrid x3 +
  * round4FFT).  xyz is atom-major (x,y,z per atom, Angstrom).  Outputs raw and
  * 2*3*5-padded sizes, origin and step per axis.  Fails with CLAT_EINVAL and

@@ -21,6 +21,36 @@ def test_round_up_tables():
     assert [clat.round_up(n, 4) for n in (0, 13, 16, 17, 34, 101)] == [4, 16, 16, 20, 36, 108]
 
 
+def _two_level(m):
+    """(R1, R2), R1 >= R2 the most balanced split of m into register sizes, or None."""
+    regs = {1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 12, 14, 15, 16, 18, 20}
+    best = None
+    for r2 in range(1, int(m ** 0.5) + 1):
+        if m % r2 == 0 and r2 in regs and m // r2 in regs:
+            best = (m // r2, r2)
+    return best
+
+
+def test_engine_length_keeps_reference_grids():
+    """Every grid length the reference produces (Round::up<2,3,5>, as
+    clat.round_up) from 32 to 400 is computed on as it is when it splits
+    into two register sizes, and on the smallest length at or above it that
+    splits otherwise; the result always splits, is never shorter, and for
+    lengths that do not split lies no further than the next 2*3*5 length
+    that does (brute force over all n up to 400)."""
+    splits = [m for m in range(32, 401) if clat.round_up(m, 1) == m and _two_level(m)]
+    for n in range(32, 401):
+        e = clat.engine_length(n)
+        assert e >= n and _two_level(e) is not None, (n, e)
+        if clat.round_up(n, 1) == n and _two_level(n):
+            assert e == n, (n, e)
+        nxt = min(m for m in splits if m >= n) if any(m >= n for m in splits) else None
+        if nxt is not None:
+            assert e <= nxt, (n, e, nxt)
+    assert [clat.engine_length(n) for n in (45, 100, 125, 250, 384, 49)] == [45, 100, 128, 256, 400, 50]
+    assert clat.engine_length(401) == clat.round_up(401, 1, (2, 3, 5, 7))
+
+
 @pytest.mark.skipif(REF is None, reason="reference tree absent: oracle/_ref not built")
 def test_round_up_vs_reference_build():
     for c in (1, 2, 3, 4, 7):
