@@ -163,3 +163,13 @@ def test_group_matches_per_engine_runs(monkeypatch):
             clat.compute_integrals_group([grp[0], grp[0]], lists[:2])
         for e in grp + [o_eng]:
             e.close()
+    # members of different precisions in one graph
+    mixed = [_engine(ws[0], clat.PREC_FP64), _engine(ws[1], clat.PREC_FP32X)]
+    solo = [_engine(ws[0], clat.PREC_FP64), _engine(ws[1], clat.PREC_FP32X)]
+    want = [e.compute_integrals(q) for e, q in zip(solo, lists[:2])]
+    for _ in range(3):
+        got = clat.compute_integrals_group(mixed, lists[:2])
+        assert all(np.array_equal(g, w_) for g, w_ in zip(got, want))
+    assert mixed[0].tuning()["graph_replays"] >= 2
+    for e in mixed + solo:
+        e.close()
