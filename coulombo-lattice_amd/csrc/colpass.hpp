@@ -369,10 +369,17 @@ __global__ __launch_bounds__((col_threads<C, R1>())) void k_colpass(const C2<TI>
   }
 }
 
-// columns per block: 128-byte rows of the arithmetic type
+// columns per block for a pass of length L: 16 (128-byte rows of complex64);
+// complex128 passes of up to 256 points take 16 as well (256-byte rows: the
+// FP64 y passes of the 1M-atom grid run at 5.4-6.2 TB/s against 4.4-5.2 with
+// 8 columns, the whole FP64 step 18.4 against 19.4 ms), longer ones 8 (their
+// LDS tile would hold a block per CU)
+__host__ __device__ constexpr int col_c_bytes(size_t c2bytes, int L) {
+  return c2bytes == 8 ? 16 : (L <= 256 ? 16 : 8);
+}
 template <typename TC>
-constexpr int col_c() {
-  return sizeof(TC) == 4 ? 16 : 8;
+constexpr int col_c(int L) {
+  return col_c_bytes(sizeof(C2<TC>), L);
 }
 
 struct ColLaunch {
@@ -387,12 +394,12 @@ struct ColLaunch {
 
 template <int R1, int R2, typename TC>
 constexpr size_t col_lds() {
-  return (size_t)R1 * R2 * col_c<TC>() * sizeof(C2<TC>);
+  return (size_t)R1 * R2 * col_c<TC>(R1 * R2) * sizeof(C2<TC>);
 }
 
 template <typename TI, typename TC, typename TO, int MODE, int R1, int R2, int NSUB>
 hipError_t col_go(const ColLaunch& L) {
-  constexpr int C = col_c<TC>();
+  constexpr int C = col_c<TC>(R1 * R2);
   constexpr int nth = col_threads<C, R1>();
   constexpr size_t lds = col_lds<R1, R2, TC>();
   hipLaunchKernelGGL((k_colpass<TI, TC, TO, MODE, R1, R2, NSUB, C>), L.grid, dim3(nth), lds, L.stream, (const C2<TI>*)L.in, (C2<TO>*)L.out, L.a,

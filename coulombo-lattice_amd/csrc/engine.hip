@@ -454,7 +454,7 @@ bool launch_col(clat_engine* e, const PassCfg& pc, const P2Args& a, int n_outer,
              !col_fits(a.L, a.nsub, a.in_is, sizeof(C2<TI>), a.out_is, sizeof(C2<TO>))) {
     return false;
   }
-  const int C = col_c<TC>();
+  const int C = col_c<TC>(pc.R1 * pc.R2);
   ColLaunch L;
   const unsigned ncb = (unsigned)((a.W + C - 1) / C);
   const unsigned nbz = (unsigned)std::max(1, a.nb);
@@ -709,7 +709,7 @@ void prepare_fast(clat_engine* e, const double2* sites_in, double2* sites_out) {
     // of column strip j at fy Z W + j Z C + z C
     const bool bt = e->btile;
     const long long Cb = bt ? e->gc_c : 0;
-    if (bt && (col_c<TC>() != Cb || col_c<TCZ>() != Cb || col_c<TCY>() != Cb))
+    if (bt && (col_c<TC>(Yi) != Cb || col_c<TCZ>(Zi) != Cb || col_c<TCY>(Yi) != Cb))
       throw host::Error(CLAT_EDEVICE, "tiled B with column passes of different widths");
     const long long bz = bt ? Cb : W;       // B: z stride
     const long long bcs = bt ? Zi * Cb : 0; // B: column-strip stride (0: C)
@@ -1932,7 +1932,7 @@ int clat_create(int X, int Y, int Z, int precision, int schedule, clat_engine** 
       e->conv_tiled = e->colk && col_pair_compiled(e->czc.R1, e->czc.R2) &&
                       col_fits(e->Zi, nsub, zrow, zb, zrow, zb);
       if (e->conv_tiled) {
-        e->gc_c = tcz == flt ? col_c<float>() : col_c<double>();
+        e->gc_c = col_c_bytes(tcz, e->Zi);
         e->gc_npar = e->shared() ? 1 : 2;
         e->gc_w = e->shared() ? 2 * e->Xi : e->Xi;
         e->gc_ncb = (e->gc_w + e->gc_c - 1) / e->gc_c;
@@ -1949,6 +1949,7 @@ int clat_create(int X, int Y, int Z, int precision, int schedule, clat_engine** 
       // passes on column kernels with one column count C.
       const long long W = 2LL * e->Xi;
       e->btile = e->shared() && e->conv_tiled && tc == tcz && tcy == tcz && W % e->gc_c == 0 &&
+                 col_c_bytes(tc, e->Yi) == e->gc_c &&
                  col_pair_compiled(e->cyf.R1, e->cyf.R2) && col_pair_compiled(e->cyi.R1, e->cyi.R2) &&
                  col_fits(e->Yi, 2, W, e->s_a, (long long)e->Zi * W, e->s_b) &&
                  col_fits(e->Yi, 2, (long long)e->Zi * W, e->s_b, W, e->s_d);
