@@ -59,6 +59,37 @@ template <typename T>
 __device__ __forceinline__ void stg_off(T* base, uint32_t byte_off, T v) {
   *reinterpret_cast<T*>(reinterpret_cast<char*>(base) + byte_off) = v;
 }
+// nontemporal store through a 64-bit address (the y forward's output)
+template <typename T>
+__device__ __forceinline__ void stg_nt(T* base, uint32_t byte_off, T v) {
+  typedef double d2v __attribute__((ext_vector_type(2)));
+  static_assert(sizeof(T) == 8 || sizeof(T) == 16, "complex64 or complex128 element");
+  T* p = reinterpret_cast<T*>(reinterpret_cast<char*>(base) + byte_off);
+  if constexpr (sizeof(T) == 8) {
+    unsigned long long u;
+    __builtin_memcpy(&u, &v, 8);
+    __builtin_nontemporal_store(u, reinterpret_cast<unsigned long long*>(p));
+  } else {
+    d2v u;
+    __builtin_memcpy(&u, &v, 16);
+    __builtin_nontemporal_store(u, reinterpret_cast<d2v*>(p));
+  }
+}
+// Cache policy of each pass's stores (buffer-instruction policy bits: 0
+// default, 2 nontemporal).  Measured on the 1M-atom step, same box, two or
+// three alternations each.  FP32X: the x forward (A), y forward (B) and z conv
+// (B in place) storing nontemporal and the y inverse (D) with the default
+// policy gives 9.75-9.78 ms per step, everything default 10.05-10.09; with
+// the y inverse nontemporal too the x inverse reading D slows from 0.60 to
+// 0.69 ms (D of four prepares, 268 MB, is partly still in the 256 MB
+// last-level cache when it starts).  The y forward's nontemporal stores take
+// a 64-bit address: as buffer stores with the same policy the pass ran
+// 2.01-2.08 ms per step against 1.89-1.90.  FP64 (complex128): the same
+// policies except a default-policy y forward give 18.14-18.18 ms against
+// 18.36-18.43 all default; a nontemporal complex128 y forward took that pass
+// from 3.7 to 4.2-4.4 ms, and the x forward with the default policy slowed
+// the y forward reading A from 3.73 to 3.98 ms.
+constexpr int ST_CONV = 2, ST_YINV = 0, ST_XFWD = 2;
 // streaming load (read once by this pass): nontemporal hint, so the line does
 // not displace reused data in the caches; measured 16-26 % faster y passes
 template <typename T>
@@ -112,7 +143,7 @@ __device__ __forceinline__ T ldg_stream_off(const T* base, uint32_t byte_off) {
   }
   return r;
 }
-template <typename T>
+template <int POL = 0, typename T>
 __device__ __forceinline__ void stg_buf(T* base, uint32_t byte_off, T v) {
   static_assert(sizeof(T) == 8 || sizeof(T) == 16, "complex64 or complex128 element");
   const __amdgpu_buffer_rsrc_t rs = col_rsrc(base);
@@ -120,12 +151,12 @@ __device__ __forceinline__ void stg_buf(T* base, uint32_t byte_off, T v) {
     typedef int i2 __attribute__((ext_vector_type(2)));
     i2 u;
     __builtin_memcpy(&u, &v, 8);
-    __builtin_amdgcn_raw_buffer_store_b64(u, rs, (int)byte_off, 0, 0);
+    __builtin_amdgcn_raw_buffer_store_b64(u, rs, (int)byte_off, 0, POL);
   } else {
     typedef int i4 __attribute__((ext_vector_type(4)));
     i4 u;
     __builtin_memcpy(&u, &v, 16);
-    __builtin_amdgcn_raw_buffer_store_b128(u, rs, (int)byte_off, 0, 0);
+    __builtin_amdgcn_raw_buffer_store_b128(u, rs, (int)byte_off, 0, POL);
   }
 }
 
@@ -306,7 +337,8 @@ __global__ __launch_bounds__((col_threads<C, R1>())) void k_colpass(const C2<TI>
           const uint32_t k1s = (uint32_t)(R2 * NSUB) * a.out_is_lo;
 #pragma unroll
           for (int k1 = 0; k1 < R1; ++k1) {
-            stg_off(bout, rb + suni((uint32_t)k1 * k1s), cvt<TC, TO>(v[k1]));
+            if constexpr (sizeof(C2<TO>) == 8) stg_nt(bout, rb + suni((uint32_t)k1 * k1s), cvt<TC, TO>(v[k1]));
+            else stg_off(bout, rb + suni((uint32_t)k1 * k1s), cvt<TC, TO>(v[k1]));
           }
         }
       } else {
@@ -364,7 +396,7 @@ __global__ __launch_bounds__((col_threads<C, R1>())) void k_colpass(const C2<TI>
 #pragma unroll
     for (int n2 = 0; n2 < R2; ++n2) {
       const uint32_t off = (ok && ((live >> n2) & 1u)) ? rbase + suni((uint32_t)n2 * a.out_is_hi) : COL_OOB;
-      stg_buf(bout, off, cvt<TC, TO>(acc[n2]));
+      stg_buf<MODE == PM_INV ? ST_YINV : ST_CONV>(bout, off, cvt<TC, TO>(acc[n2]));
     }
   }
 }

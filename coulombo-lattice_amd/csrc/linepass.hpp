@@ -191,7 +191,7 @@ __global__ __launch_bounds__((line_threads<C, R1, R2, NSUB>())) void k_linepass(
       freq_offs<R1, R2 * NSUB>(a, k2 * NSUB + s2, off);
 #pragma unroll
       for (int k1 = 0; k1 < R1; ++k1)
-        stg_buf(ob, (uint32_t)(lb + off[k1]) * (uint32_t)sizeof(C2<TO>), cvt<TC, TO>(w[k1]));
+        stg_buf<ST_XFWD>(ob, (uint32_t)(lb + off[k1]) * (uint32_t)sizeof(C2<TO>), cvt<TC, TO>(w[k1]));
     }
   } else {
     // ---------------------------------------------------------- P2 <- D
