@@ -49,6 +49,9 @@ struct ColArgs2 {
   // > 0: the batched prepares are folded into blockIdx.x = prepare * nxb +
   // column block (nxb column blocks), instead of blockIdx.z
   int nxb;
+  // coupled conv of the two-coset path: element offset of the second coset's
+  // plane from the first (input and output)
+  long long cset;
 };
 
 template <typename T>
@@ -401,6 +404,200 @@ __global__ __launch_bounds__((col_threads<C, R1>())) void k_colpass(const C2<TI>
   }
 }
 
+// Coupled z conv of the two-coset path.  The sites of the grid lie on two
+// cosets of the coarse lattice 2Z^3 (x = y = z = 0 and = 1 mod 2); each coset
+// is a plane of the coarse grid, and V on coset i is the sum over cosets j of
+// the open-boundary convolution of rho_j with G sampled at offsets
+// 2 d + t_i - t_j.  Per frequency the two transformed columns are mixed by
+//   [V0; V1] = [g0, conj(gt); gt, g0] [R0; R1]
+// with g0 = the kernel within a coset (real, the conv-table tile of the
+// single-plane pass) and gt = the kernel from coset 0 to coset 1,
+//   gt(fx, fy, fz) = Q[r(fy)][column block][fz][c] * Py[fy]
+// (Q holds the x and z phases, Py the y phase and the sign of the y
+// reflection).  Same blocks, index maps, strips and batching as the conv
+// mode of k_colpass; the second coset's plane is a.cset elements after the
+// first, in and out.
+template <typename TI, typename TC, typename TO, int R1, int R2, int C>
+__global__ __launch_bounds__((col_threads<C, R1>())) void k_colconv2(const C2<TI>* in, C2<TO>* out, ColArgs2 a,
+                                                                   const C2<TC>* __restrict__ T2L,
+                                                                   const TC* __restrict__ G,
+                                                                   const C2<TC>* __restrict__ Q,
+                                                                   const C2<TC>* __restrict__ Py) {
+  using CT = C2<TC>;
+  constexpr int L = R1 * R2, NSUB = 2;
+  constexpr uint32_t SI = sizeof(C2<TI>), SO = sizeof(C2<TO>), SC = sizeof(CT);
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  CT* lds0 = reinterpret_cast<CT*>(smem_raw);
+  CT* lds1 = lds0 + L * C;
+  const int tid = threadIdx.x;
+  const int c = tid & (C - 1);
+  const int r = tid / C;
+  int cbk = (int)blockIdx.x;
+  int bz = (int)blockIdx.z;
+  if (a.nxb > 0) {
+    bz = cbk / a.nxb;
+    cbk -= bz * a.nxb;
+  }
+  const int col = cbk * C + c;
+  const bool okc = col < a.W;
+  // fy and its reflection 2Y - fy in adjacent blocks (same table tiles)
+  int o = (int)blockIdx.y;
+  if (a.pair) {
+    const int j = o, last = (int)gridDim.y - 1;
+    const int t = (j + 1) >> 1;
+    o = (j == 0 || j == last) ? (j == 0 ? 0 : a.Y2 >> 1) : ((j & 1) ? t : a.Y2 - t);
+  }
+  const C2<TI>* bin0 = in + (size_t)bz * a.in_bs + (size_t)o * a.in_os_hi + (size_t)cbk * a.in_cbs;
+  const C2<TI>* bin1 = bin0 + a.cset;
+  C2<TO>* bout0 = out + (size_t)bz * a.out_bs + (size_t)o * a.out_os_hi + (size_t)cbk * a.out_cbs;
+  C2<TO>* bout1 = bout0 + a.cset;
+  const uint32_t cin = (uint32_t)c * SI, cout = (uint32_t)c * SO;
+
+  constexpr int NTH = col_threads<C, R1>();
+  constexpr int NTW = (2 * L + NTH - 1) / NTH;
+  __shared__ CT twl[2 * L];
+  CT twv[NTW];
+#pragma unroll
+  for (int j = 0; j < NTW; ++j) twv[j] = T2L[min(tid + j * NTH, 2 * L - 1)];
+
+  // thread (c, n1 = r) loads x_i[n1 + R1 n2], n2 < R2, of both cosets
+  CT x0[R2], x1[R2];
+  {
+    const bool ok = okc && r < R1;
+    const uint32_t rbase = (uint32_t)r * a.in_is_lo + cin;
+#pragma unroll
+    for (int n2 = 0; n2 < R2; ++n2) {
+      const uint32_t off = ok ? rbase + suni((uint32_t)n2 * a.in_is_hi) : COL_OOB;
+      x0[n2] = cvt<TI, TC>(ldg_stream_off(bin0, off));
+      x1[n2] = cvt<TI, TC>(ldg_stream_off(bin1, off));
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < NTW; ++j)
+    if (tid + j * NTH < 2 * L) twl[tid + j * NTH] = twv[j];
+  __syncthreads();
+  auto TW = [&](uint32_t byte_off) {
+    return *reinterpret_cast<const CT*>(reinterpret_cast<const char*>(twl) + byte_off);
+  };
+
+  CT acc0[R2], acc1[R2];
+  // table tiles of the block (block-uniform bases, the column in the offset)
+  const int fy = o * a.gym + a.gya;
+  const int ry = min(fy, a.Y2 - fy);
+  const TC* Gb = G + ((size_t)(ry * a.npar + a.par) * a.ncb + cbk) * a.GZ1 * C;
+  const CT* Qb = Q + ((size_t)ry * a.ncb + cbk) * (size_t)(2 * L) * C;
+  const uint32_t gcol = (uint32_t)c * (uint32_t)sizeof(TC), qcol = (uint32_t)c * SC;
+  const CT py = Py[fy];
+
+#pragma unroll
+  for (int s = 0; s < NSUB; ++s) {
+    // -------------------------------------------------------------- P1
+    if (r < R1) {
+      CT v[R2], w[R2];
+#pragma unroll
+      for (int n2 = 0; n2 < R2; ++n2) {
+        v[n2] = x0[n2];
+        w[n2] = x1[n2];
+      }
+      if (s == 1) {
+        rot_all<R2, false, TC>(v);
+        rot_all<R2, false, TC>(w);
+      }
+      rdft<R2, false, TC>(v);
+      rdft<R2, false, TC>(w);
+      const uint32_t tstep = (uint32_t)(2 * r) * SC;
+      const uint32_t tbase = (uint32_t)(s * r) * SC;
+#pragma unroll
+      for (int k2 = 0; k2 < R2; ++k2) {
+        CT t0 = v[k2], t1 = w[k2];
+        if (k2 > 0 || s == 1) {
+          const CT tw = TW(tbase + (uint32_t)k2 * tstep);
+          t0 = cmul<TC>(t0, tw);
+          t1 = cmul<TC>(t1, tw);
+        }
+        lds0[(k2 * R1 + r) * C + c] = t0;
+        lds1[(k2 * R1 + r) * C + c] = t1;
+      }
+    }
+    __syncthreads();
+    // -------------------------------------------------------------- P2
+    if (r < R2) {
+      CT v[R1], w[R1];
+#pragma unroll
+      for (int n1 = 0; n1 < R1; ++n1) {
+        v[n1] = lds0[(r * R1 + n1) * C + c];
+        w[n1] = lds1[(r * R1 + n1) * C + c];
+      }
+      rdft<R1, false, TC>(v);  // v[k1] = R0_s[R2 k1 + k2]
+      rdft<R1, false, TC>(w);
+      // fz = 2 (R2 k1 + k2) + s; g0 row rz = min(fz, 2L - fz), Q row fz
+      const int q = 2 * r + s;
+      const uint32_t lo = (uint32_t)q * (C * sizeof(TC));
+#pragma unroll
+      for (int k1 = 0; k1 < R1; ++k1) {
+        constexpr uint32_t RB = C * sizeof(TC);
+        uint32_t off;
+        if (2 * R2 * (k1 + 1) - 1 <= L) off = lo + (uint32_t)(2 * R2 * k1) * RB;
+        else if (2 * R2 * k1 >= L) off = (uint32_t)(2 * L - 2 * R2 * k1) * RB - lo;
+        else {
+          const int fz = 2 * R2 * k1 + q;
+          off = (uint32_t)min(fz, 2 * L - fz) * RB;
+        }
+        const TC g = ldg_off(Gb, off + gcol);
+        const CT gt = cmul<TC>(ldg_off(Qb, (uint32_t)(2 * R2 * k1 + q) * (C * SC) + qcol), py);
+        const CT a0 = v[k1], a1 = w[k1];
+        v[k1] = mk<TC>(g * a0.x + gt.x * a1.x + gt.y * a1.y, g * a0.y + gt.x * a1.y - gt.y * a1.x);
+        w[k1] = mk<TC>(gt.x * a0.x - gt.y * a0.y + g * a1.x, gt.x * a0.y + gt.y * a0.x + g * a1.y);
+      }
+      rdft<R1, true, TC>(v);
+      rdft<R1, true, TC>(w);
+      const uint32_t tstep = (uint32_t)(2 * r + s) * SC;
+#pragma unroll
+      for (int n1 = 0; n1 < R1; ++n1) {
+        CT t0 = v[n1], t1 = w[n1];
+        if (n1 > 0) {
+          const CT tw = TW((uint32_t)n1 * tstep);
+          t0 = cmulc<TC>(t0, tw);
+          t1 = cmulc<TC>(t1, tw);
+        }
+        lds0[(r * R1 + n1) * C + c] = t0;
+        lds1[(r * R1 + n1) * C + c] = t1;
+      }
+    }
+    __syncthreads();
+    // -------------------------------------------------------------- P3
+    if (r < R1) {
+      CT v[R2], w[R2];
+#pragma unroll
+      for (int k2 = 0; k2 < R2; ++k2) {
+        v[k2] = lds0[(k2 * R1 + r) * C + c];
+        w[k2] = lds1[(k2 * R1 + r) * C + c];
+      }
+      rdft<R2, true, TC>(v);
+      rdft<R2, true, TC>(w);
+      if (s == 1) {
+        rot_all<R2, true, TC>(v);
+        rot_all<R2, true, TC>(w);
+      }
+#pragma unroll
+      for (int n2 = 0; n2 < R2; ++n2) {
+        acc0[n2] = s == 0 ? v[n2] : cadd<TC>(acc0[n2], v[n2]);
+        acc1[n2] = s == 0 ? w[n2] : cadd<TC>(acc1[n2], w[n2]);
+      }
+    }
+    if (s + 1 < NSUB) __syncthreads();
+  }
+
+  const bool ok = okc && r < R1;
+  const uint32_t rbase = (uint32_t)r * a.out_is_lo + cout;
+#pragma unroll
+  for (int n2 = 0; n2 < R2; ++n2) {
+    const uint32_t off = ok ? rbase + suni((uint32_t)n2 * a.out_is_hi) : COL_OOB;
+    stg_buf<ST_CONV>(bout0, off, cvt<TC, TO>(acc0[n2]));
+    stg_buf<ST_CONV>(bout1, off, cvt<TC, TO>(acc1[n2]));
+  }
+}
+
 // columns per block for a pass of length L: 16 (128-byte rows of complex64);
 // complex128 passes of up to 256 points take 16 as well (256-byte rows: the
 // FP64 y passes of the 1M-atom grid run at 5.4-6.2 TB/s against 4.4-5.2 with
@@ -422,6 +619,8 @@ struct ColLaunch {
   ColArgs2 a;
   const void* T2L;
   const void* G;
+  const void* Q;   // coupled conv: the cross-coset table and the y factors
+  const void* Py;
 };
 
 template <int R1, int R2, typename TC>
@@ -439,6 +638,27 @@ hipError_t col_go(const ColLaunch& L) {
   return hipGetLastError();
 }
 
+// LDS of the coupled conv: one exchange tile per coset
+template <int R1, int R2, typename TC>
+constexpr size_t conv2_lds() {
+  return 2 * col_lds<R1, R2, TC>();
+}
+
+template <typename TI, typename TC, typename TO, int R1, int R2>
+hipError_t conv2_go(const ColLaunch& L) {
+  constexpr int C = col_c<TC>(R1 * R2);
+  constexpr int nth = col_threads<C, R1>();
+  constexpr size_t lds = conv2_lds<R1, R2, TC>();
+  if constexpr (lds + 2 * R1 * R2 * sizeof(C2<TC>) > 65536) {
+    return hipErrorInvalidValue;  // the host keeps such grids on the single-plane path
+  } else {
+    hipLaunchKernelGGL((k_colconv2<TI, TC, TO, R1, R2, C>), L.grid, dim3(nth), lds, L.stream, (const C2<TI>*)L.in,
+                       (C2<TO>*)L.out, L.a, (const C2<TC>*)L.T2L, (const TC*)L.G, (const C2<TC>*)L.Q,
+                       (const C2<TC>*)L.Py);
+    return hipGetLastError();
+  }
+}
+
 template <typename TI, typename TC, typename TO, int MODE, int R1, int R2>
 hipError_t col_go_n(int nsub, const ColLaunch& L) {
   return nsub == 2 ? col_go<TI, TC, TO, MODE, R1, R2, 2>(L) : col_go<TI, TC, TO, MODE, R1, R2, 1>(L);
@@ -453,14 +673,17 @@ hipError_t col_launch(int combo, int mode, int nsub, const ColLaunch& L) {
     case CB_DDD:
       if (mode == PM_FWD) return col_go_n<d, d, d, PM_FWD, R1, R2>(nsub, L);
       if (mode == PM_CONV) return col_go_n<d, d, d, PM_CONV, R1, R2>(nsub, L);
+      if (mode == PM_CONV2) return conv2_go<d, d, d, R1, R2>(L);
       return col_go_n<d, d, d, PM_INV, R1, R2>(nsub, L);
     case CB_FFF:
       if (mode == PM_FWD) return col_go_n<f, f, f, PM_FWD, R1, R2>(nsub, L);
       if (mode == PM_CONV) return col_go_n<f, f, f, PM_CONV, R1, R2>(nsub, L);
+      if (mode == PM_CONV2) return conv2_go<f, f, f, R1, R2>(L);
       return col_go_n<f, f, f, PM_INV, R1, R2>(nsub, L);
     case CB_FDF:
       if (mode == PM_FWD) return col_go_n<f, d, f, PM_FWD, R1, R2>(nsub, L);
       if (mode == PM_CONV) return col_go_n<f, d, f, PM_CONV, R1, R2>(nsub, L);
+      if (mode == PM_CONV2) return conv2_go<f, d, f, R1, R2>(L);
       break;
     case CB_FDD:
       if (mode == PM_INV) return col_go_n<f, d, d, PM_INV, R1, R2>(nsub, L);

@@ -304,6 +304,24 @@ struct clat_engine {
   int64_t graph_replays = 0, graph_captures = 0;
   int64_t serial = 0;                   // unique per engine of the process (group graph keys)
   int n_cu = 256;     // compute units of the device
+  // Two-coset path of the atom path (shared schedule, line kernels): when
+  // every site lies on x = y = z (mod 2) of an even grid, the sites split
+  // into the cosets 2m and 2m + (1,1,1) of the coarse grid with half the
+  // points per axis.  The coarse engine `sub` (on this engine's stream) runs
+  // the five passes on the two coset planes of each prepare, its conv
+  // coupling them through the 2 x 2 kernel (k_colconv2); sites are ordered
+  // coset 0, then coset 1, each over the n_u coarse cells either occupies.
+  clat_engine* sub = nullptr;
+  bool sub_failed = false;      // the coarse engine cannot run this grid
+  bool coset = false;           // the current atoms run on the two-coset path
+  int64_t n_u = 0;
+  // kept by the coarse engine of a two-coset path
+  clat_engine* kparent = nullptr;  // kernel timing goes to this engine's events
+  bool own_stream = true;
+  bool pair_conv = false;          // the conv couples planes 2b and 2b + 1
+  void* Q = nullptr;               // cross-coset kernel table (k_coarse_gt)
+  size_t q_elems = 0;
+  void* Py = nullptr;              // y phase and reflection sign of Q per fy
   // host buffers of a run, kept with their pages between runs
   std::vector<host::PlanStep> plan_steps;
   std::vector<char> plan_in_cover, plan_leads;
@@ -357,17 +375,21 @@ struct clat_engine {
   size_t NU() const { return (size_t)X * Y * Z; }
   bool shared() const { return sched == CLAT_SCHED_SHARED; }
 
-  bool ktimed(int kind) const { return (kmask >> kind) & 1u; }
+  // kernel timing: a coarse engine records into its parent's events
+  clat_engine* kown() { return kparent ? kparent : this; }
+  bool ktimed(int kind) const { return ((kparent ? kparent->kmask : kmask) >> kind) & 1u; }
   void kbegin(int kind) {
     if (!ktimed(kind)) return;
-    ensure_events(kused + 2);
-    HIPX(hipEventRecord(kev[kused], stream));
+    clat_engine* o = kown();
+    o->ensure_events(o->kused + 2);
+    HIPX(hipEventRecord(o->kev[o->kused], stream));
   }
   void kend(int kind, double bytes) {
     if (!ktimed(kind)) return;
-    HIPX(hipEventRecord(kev[kused + 1], stream));
-    kpending.push_back({kind, bytes});
-    kused += 2;
+    clat_engine* o = kown();
+    HIPX(hipEventRecord(o->kev[o->kused + 1], stream));
+    o->kpending.push_back({kind, bytes});
+    o->kused += 2;
   }
   void ensure_events(size_t n) {
     while (kev.size() < n) {
@@ -492,8 +514,10 @@ bool launch_col(clat_engine* e, const PassCfg& pc, const P2Args& a, int n_outer,
   c.out_cbs = a.out_cbs ? a.out_cbs : C;
   c.outer_x = a.outer_x;
   c.nxb = fold ? (int)ncb : 0;
+  c.cset = 0;
   L.T2L = sizeof(TC) == 8 ? (const void*)e->dtw[axis] : (const void*)e->dtw32[axis];
   L.G = gc_of<TC>(e);
+  L.Q = L.Py = nullptr;
   constexpr int combo = combo_of<TI, TC, TO>();
   e->kbegin(kind);
   hipError_t err = hipErrorInvalidValue;
@@ -507,6 +531,63 @@ bool launch_col(clat_engine* e, const PassCfg& pc, const P2Args& a, int n_outer,
   HIPX(err);
   e->kend(kind, bytes);
   return true;
+}
+
+// Coupled conv of a coarse two-coset engine: a.nb prepares a.in_bs elements
+// apart, each with its coset planes e->b_set apart in B; otherwise the
+// launch shape of the conv on the column kernel (fy pairs, prepares folded
+// into blockIdx.x).
+template <typename TB, typename TC>
+void launch_conv2(clat_engine* e, const PassCfg& pc, const P2Args& a, int n_outer, double bytes) {
+  if (!e->Q || !e->Py || (sizeof(TC) == 4) != (e->Gc32 != nullptr))
+    throw host::Error(CLAT_EDEVICE, "coupled conv without its kernel tables");
+  const int C = col_c<TC>(pc.R1 * pc.R2);
+  ColLaunch L;
+  const unsigned ncb = (unsigned)((a.W + C - 1) / C);
+  const unsigned nbz = (unsigned)std::max(1, a.nb);
+  const bool fold = nbz > 1 && ncb % 8 == 0;
+  L.grid = fold ? dim3(ncb * nbz, (unsigned)n_outer, 1) : dim3(ncb, (unsigned)n_outer, nbz);
+  L.stream = e->stream;
+  L.in = e->B;
+  L.out = e->B;
+  ColArgs2& c = L.a;
+  std::memset(&c, 0, sizeof c);
+  c.W = a.W;
+  c.od = 1;
+  c.in_os_hi = a.in_os;
+  c.out_os_hi = a.out_os;
+  c.in_is_lo = (unsigned)(a.in_is * (long long)sizeof(C2<TB>));
+  c.out_is_lo = (unsigned)(a.out_is * (long long)sizeof(C2<TB>));
+  c.in_is_hi = c.in_is_lo * (unsigned)pc.R1;
+  c.out_is_hi = c.out_is_lo * (unsigned)pc.R1;
+  c.gym = 1;
+  c.Y2 = a.Y2;
+  c.GZ1 = a.GZ1;
+  c.ncb = e->gc_ncb;
+  c.npar = 1;
+  c.pair = 1;
+  c.in_bs = a.in_bs;
+  c.out_bs = a.out_bs;
+  c.in_cbs = a.in_cbs ? a.in_cbs : C;
+  c.out_cbs = a.out_cbs ? a.out_cbs : C;
+  c.nxb = fold ? (int)ncb : 0;
+  c.cset = (long long)e->b_set;
+  L.T2L = sizeof(TC) == 8 ? (const void*)e->dtw[2] : (const void*)e->dtw32[2];
+  L.G = gc_of<TC>(e);
+  L.Q = e->Q;
+  L.Py = e->Py;
+  constexpr int combo = combo_of<TB, TC, TB>();
+  e->kbegin(K_ZCONV);
+  hipError_t err = hipErrorInvalidValue;
+  switch (pc.R1 * 64 + pc.R2) {
+#define CLAT_C2_CASE(ra, rb) \
+  case ra * 64 + rb: err = clat_c2_##ra##x##rb(combo, PM_CONV2, 2, L); break;
+    CLAT_P2_PAIRS(CLAT_C2_CASE)
+#undef CLAT_C2_CASE
+    default: break;
+  }
+  HIPX(err);
+  e->kend(K_ZCONV, bytes);
 }
 
 template <typename TI, typename TC, typename TO, int MODE, bool LINE>
@@ -742,8 +823,19 @@ void prepare_fast(clat_engine* e, const double2* sites_in, double2* sites_out) {
       if (lk) batch(a, (long long)e->b_set, (long long)e->b_set);
       // algorithmic bytes: B read and written per prepare, the kernel-table
       // octant once per launch (the batched prepares share it)
-      launch2<TB, TCZ, TB, PM_CONV, false>(e, e->czc, a, 2 * Yi, 2, e->B, e->B, K_ZCONV,
-                                           sb * 8 * N * nb + (double)sizeof(TCZ) * (double)e->NG());
+      if (e->pair_conv) {
+        // two coset planes per prepare: nb / 2 prepares, 2 b_set apart; the
+        // cross-coset table read once per launch besides the octant
+        if (!lk || nb % 2) throw host::Error(CLAT_EDEVICE, "coupled conv outside a two-coset batch");
+        a.nb = nb / 2;
+        a.in_bs = a.out_bs = 2LL * (long long)e->b_set;
+        launch_conv2<TB, TCZ>(e, e->czc, a, 2 * Yi,
+                              sb * 8 * N * nb + (double)sizeof(TCZ) * (double)e->NG() +
+                                  (double)sizeof(C2<TCZ>) * (double)e->q_elems);
+      } else {
+        launch2<TB, TCZ, TB, PM_CONV, false>(e, e->czc, a, 2 * Yi, 2, e->B, e->B, K_ZCONV,
+                                             sb * 8 * N * nb + (double)sizeof(TCZ) * (double)e->NG());
+      }
     }
     {
       P2Args a = base_args(e);
@@ -947,8 +1039,14 @@ void prepare_dense(clat_engine* e) {
   }
 }
 
+void coset_prepare(clat_engine* e, const double2* rho_sites, double2* v_sites);
+
 // pair density at the sites -> potential at the sites (FP64)
 void prepare_sites(clat_engine* e, const double2* rho_sites, double2* v_sites) {
+  if (e->coset) {
+    coset_prepare(e, rho_sites, v_sites);
+    return;
+  }
   if (e->fast) {
     if (e->rprec == CLAT_PREC_FP64) prepare_fast<double, double, double, double, double, double, double>(e, rho_sites, v_sites);
     else if (e->rprec == CLAT_PREC_FP32) prepare_fast<float, float, float, float, float, float, float>(e, rho_sites, v_sites);
@@ -1009,6 +1107,28 @@ void dct_axis(clat_engine* e, int axis, double scale) {
   e->kend(K_DCT, 16.0 * (double)e->NG());
 }
 
+void tile_conv_table(clat_engine* e);
+
+// kernel tables of the coarse engine of a two-coset path from this engine's
+// fine octant: the within-coset octant (then tiled like any conv table) and
+// the tiled cross-coset table
+void build_coset_tables(clat_engine* e) {
+  clat_engine* s = e->sub;
+  s->stream = e->stream;
+  hipLaunchKernelGGL(k_coarse_g0, dim3(grid1d(s->NG())), dim3(256), 0, e->stream, e->G64, e->Xi, e->Yi, e->Zi,
+                     s->G64);
+  LAUNCH_CHECK();
+  tile_conv_table(s);
+  if (s->Gc32)
+    hipLaunchKernelGGL(k_coarse_gt<float>, dim3(grid1d(s->q_elems)), dim3(256), 0, e->stream, e->G64, e->Xi, e->Yi,
+                       e->Zi, s->gc_c, s->gc_ncb, (float2*)s->Q);
+  else
+    hipLaunchKernelGGL(k_coarse_gt<double>, dim3(grid1d(s->q_elems)), dim3(256), 0, e->stream, e->G64, e->Xi,
+                       e->Yi, e->Zi, s->gc_c, s->gc_ncb, (double2*)s->Q);
+  LAUNCH_CHECK();
+  s->initialized = true;
+}
+
 void finish_interaction(clat_engine* e) {
   dct_axis(e, 0, 1.0);
   dct_axis(e, 1, 1.0);
@@ -1017,6 +1137,15 @@ void finish_interaction(clat_engine* e) {
     hipLaunchKernelGGL(k_d2f, dim3(grid1d(e->NG())), dim3(256), 0, e->stream, e->G64, e->G32, e->NG());
     LAUNCH_CHECK();
   }
+  tile_conv_table(e);
+  if (e->coset) build_coset_tables(e);
+  e->initialized = true;
+  e->prepared = false;
+}
+
+// the conv pass's copy of the kernel table (tiled per column block, or the
+// row layout of the general kernel)
+void tile_conv_table(clat_engine* e) {
   if (e->conv_tiled) {
     if (e->Gc64)
       hipLaunchKernelGGL(k_gconv_tiled<double>, dim3(grid1d(e->gc_elems)), dim3(256), 0, e->stream, e->G64,
@@ -1034,8 +1163,6 @@ void finish_interaction(clat_engine* e) {
                        e->Xi, e->Yi + 1, e->Zi + 1);
     LAUNCH_CHECK();
   }
-  e->initialized = true;
-  e->prepared = false;
 }
 
 GArgs base_gargs(const clat_engine* e) {
@@ -1152,6 +1279,36 @@ struct BatchScope {
   BatchScope(const BatchScope&) = delete;
   BatchScope& operator=(const BatchScope&) = delete;
 };
+
+// The e->cur_nb prepares of a two-coset engine (sites in e->b_in / e->b_out,
+// or rho_sites / v_sites for a single prepare) on its coarse engine: the
+// coset-0 and coset-1 halves of each prepare's site arrays are planes 2b and
+// 2b + 1 of one coarse batch.
+void coset_prepare(clat_engine* e, const double2* rho_sites, double2* v_sites) {
+  clat_engine* s = e->sub;
+  const int nb = e->cur_nb;
+  if (!s || 2 * nb > LINE_NB || 2 * nb > s->nb_max)
+    throw host::Error(CLAT_EDEVICE, "two-coset batch exceeds the coarse engine's buffers");
+  // the coarse passes go wherever this engine's launches go now (a group
+  // capture points the member's stream at the lead's)
+  s->stream = e->stream;
+  if (nb == 1) {
+    e->b_in[0] = rho_sites;
+    e->b_out[0] = v_sites;
+  }
+  BatchScope guard(s);
+  s->cur_nb = 2 * nb;
+  const size_t nu = (size_t)e->n_u;
+  for (int b = 0; b < nb; ++b)
+    for (int c = 0; c < 2; ++c) {
+      const int j = 2 * b + c;
+      s->b_in[j] = e->b_in[b] + c * nu;
+      s->b_out[j] = e->b_out[b] + c * nu;
+      s->b_in2[j] = e->b_in2[b] ? e->b_in2[b] + c * nu : nullptr;
+      s->b_out2[j] = e->b_out2[b] ? e->b_out2[b] + c * nu : nullptr;
+    }
+  prepare_sites(s, s->b_in[0], s->b_out[0]);
+}
 
 // V of state f's self density at every atom's cell (potentials.cpp:140-158):
 // the self density at the sites into rho, its potential at the sites into vs,
@@ -1326,7 +1483,7 @@ std::map<std::vector<int64_t>, int> g_tuned;
 
 std::vector<int64_t> tune_key(const clat_engine* e) {
   return {e->Xi, e->Yi, e->Zi, e->rprec, e->mix, e->sched, e->nb_max, e->n_live, e->n_sites,
-          e->atile ? 1 : 0, e->btile ? 1 : 0, e->n_cu};
+          e->atile ? 1 : 0, e->btile ? 1 : 0, e->n_cu, e->coset ? e->sub->n_live : -1};
 }
 
 // Prepares per pass launch for this run (the autotuner, the counterpart of
@@ -1703,6 +1860,123 @@ void group_stage(clat_engine* const* es, int count, std::vector<RunPlan>& rps) {
   }
 }
 
+// The coarse engine of the two-coset path (created on first use): half the
+// engine points per axis, the same precision and schedule, this engine's
+// stream and kernel timing, buffers for two coset planes per prepare.  False
+// when the coarse grid cannot run every pass on the compile-time kernels,
+// or the coupled conv's two LDS tiles would exceed 64 KiB.
+bool ensure_sub(clat_engine* e) {
+  if (e->sub) return true;
+  if (e->sub_failed) return false;
+  clat_engine* s = nullptr;
+  if (clat_create(e->Xi / 2, e->Yi / 2, e->Zi / 2, e->prec, e->sched, &s) != CLAT_OK) {
+    e->sub_failed = true;
+    return false;
+  }
+  const size_t csz = s->Gc32 ? sizeof(float2) : sizeof(double2);
+  const size_t conv2_lds = (2 * (size_t)s->Zi * s->gc_c + 2 * (size_t)s->Zi) * csz;
+  const bool ok = 2 * s->Xi == e->Xi && 2 * s->Yi == e->Yi && 2 * s->Zi == e->Zi && s->xline && s->conv_tiled &&
+                  s->btile && s->atile && s->rprec == e->rprec && s->mix == e->mix && conv2_lds <= 65536 &&
+                  2 * e->nb_max <= LINE_NB;
+  if (!ok) {
+    clat_destroy(s);
+    e->sub_failed = true;
+    return false;
+  }
+  try {
+    HIPX(hipStreamSynchronize(s->stream));
+    hipStream_t own = s->stream;
+    s->stream = e->stream;
+    s->own_stream = false;
+    HIPX(hipStreamDestroy(own));
+    s->kparent = e;
+    s->pair_conv = true;
+    s->tune = 0;
+    // plane buffers for every prepare count the parent may batch
+    if (s->D != s->A) s->dfree(s->D, s->d_bytes);
+    s->D = nullptr;
+    s->dfree(s->A, s->a_bytes);
+    s->dfree(s->B, s->b_bytes);
+    s->nb_max = s->nb_rule = 2 * e->nb_max;
+    s->a_bytes = s->nb_max * s->a_set * s->s_a;
+    s->b_bytes = s->nb_max * s->b_set * s->s_b;
+    s->A = s->dalloc(s->a_bytes);
+    s->B = s->dalloc(s->b_bytes);
+    if (s->s_d == s->s_a) {
+      s->D = s->A;
+    } else {
+      s->d_bytes = s->nb_max * s->d_set * s->s_d;
+      s->D = s->dalloc(s->d_bytes);
+    }
+    // cross-coset table and the per-fy factor of its y reflection:
+    // Py[fy] = sign * e^{i pi fy / (2 Yc)}, sign -1 above fy = Yc
+    s->q_elems = (size_t)(s->Yi + 1) * s->gc_ncb * (2 * s->Zi) * s->gc_c;  // rows fz in [0, 2 Zc)
+    s->Q = s->dalloc(s->q_elems * csz);
+    const int ny = 2 * s->Yi;
+    std::vector<double> py((size_t)2 * ny);
+    for (int fy = 0; fy < ny; ++fy) {
+      const double sg = fy > s->Yi ? -1.0 : 1.0, ang = M_PI * (double)fy / (double)ny;
+      py[(size_t)2 * fy] = sg * std::cos(ang);
+      py[(size_t)2 * fy + 1] = sg * std::sin(ang);
+    }
+    s->Py = s->dalloc((size_t)ny * csz);
+    if (csz == sizeof(double2)) {
+      HIPX(hipMemcpy(s->Py, py.data(), py.size() * sizeof(double), hipMemcpyHostToDevice));
+    } else {
+      std::vector<float> pf(py.begin(), py.end());
+      HIPX(hipMemcpy(s->Py, pf.data(), pf.size() * sizeof(float), hipMemcpyHostToDevice));
+    }
+  } catch (...) {
+    clat_destroy(s);
+    e->sub_failed = true;
+    throw;
+  }
+  e->sub = s;
+  return true;
+}
+
+// Sites of the two-coset path: coset 0 (all coordinates even), then coset 1,
+// each over the ascending coarse cells (of the coarse grid Xi/2 x Yi/2 x
+// Zi/2) that either coset occupies; a coarse cell occupied in one coset only
+// is a site without atom in the other (zero coefficients, so zero density;
+// its potential is never read).  False when a site lies off x = y = z (mod 2).
+struct CosetOrder {
+  std::vector<int32_t> ucell;              // coarse cell of each of the n_u positions
+  std::vector<int32_t> site_atom, site_cell;  // 2 n_u sites: atom (or -1), engine cell
+  std::vector<int32_t> new_of;             // cell-ordered site -> two-coset site
+};
+
+bool coset_order(const clat_engine* e, const std::vector<int32_t>& site_atom, const std::vector<int32_t>& site_cell,
+                 CosetOrder& co) {
+  const int Xi = e->Xi, Yi = e->Yi, Xc = e->Xi / 2, Yc = e->Yi / 2, Zc = e->Zi / 2;
+  const int64_t nc = (int64_t)Xc * Yc * Zc;
+  std::vector<int32_t> slot((size_t)(2 * nc), -1);  // (coset, coarse cell) -> cell-ordered site
+  for (size_t k = 0; k < site_cell.size(); ++k) {
+    const int64_t cell = site_cell[k], line = cell / Xi;
+    const int ix = (int)(cell - line * Xi), iy = (int)(line % Yi), iz = (int)(line / Yi);
+    const int p = ix & 1;
+    if ((iy & 1) != p || (iz & 1) != p) return false;
+    slot[(size_t)(p * nc + ((int64_t)(iz >> 1) * Yc + (iy >> 1)) * Xc + (ix >> 1))] = (int32_t)k;
+  }
+  for (int64_t u = 0; u < nc; ++u)
+    if (slot[(size_t)u] >= 0 || slot[(size_t)(nc + u)] >= 0) co.ucell.push_back((int32_t)u);
+  const size_t nu = co.ucell.size();
+  co.site_atom.assign(2 * nu, -1);
+  co.site_cell.assign(2 * nu, 0);
+  co.new_of.assign(site_cell.size(), -1);
+  for (int p = 0; p < 2; ++p)
+    for (size_t r = 0; r < nu; ++r) {
+      const int64_t u = co.ucell[r];
+      const int mx = (int)(u % Xc), my = (int)((u / Xc) % Yc), mz = (int)(u / ((int64_t)Xc * Yc));
+      const size_t j = (size_t)p * nu + r;
+      const int32_t k = slot[(size_t)(p * nc + u)];
+      co.site_atom[j] = k >= 0 ? site_atom[(size_t)k] : -1;
+      co.site_cell[j] = (int32_t)(((int64_t)(2 * mz + p) * Yi + (2 * my + p)) * Xi + (2 * mx + p));
+      if (k >= 0) co.new_of[(size_t)k] = (int32_t)j;
+    }
+  return true;
+}
+
 void free_atoms(clat_engine* e) {
   e->dfree(e->d_site_atom, (size_t)e->n_sites * 4);
   e->dfree(e->d_site_cell, (size_t)e->n_sites * 4);
@@ -1721,6 +1995,8 @@ void free_atoms(clat_engine* e) {
   e->n_states = e->cap_states = 0;
   e->n_sites = 0;
   e->n_atoms = 0;
+  e->coset = false;
+  e->n_u = 0;
 }
 
 }  // namespace
@@ -2021,12 +2297,17 @@ int clat_create(int X, int Y, int Z, int precision, int schedule, clat_engine** 
 int clat_destroy(clat_engine* e) {
   if (!e) return CLAT_OK;
   if (e->stream) (void)hipStreamSynchronize(e->stream);
+  if (e->sub) {
+    e->sub->stream = e->stream;  // (a group capture may have left it on another engine's stream)
+    (void)clat_destroy(e->sub);
+    e->sub = nullptr;
+  }
   void* ptrs[] = {e->tw32[0], e->tw32[1], e->tw32[2], e->h32[0], e->h32[1], e->h32[2], e->tw64[0], e->tw64[1],
                   e->tw64[2], e->h64[0], e->h64[1], e->h64[2], e->dtw[0], e->dtw[1], e->dtw[2], e->dtw32[0], e->dtw32[1], e->dtw32[2], e->G64,
                   e->G32, e->Gc64, e->Gc32, e->F, e->V, e->A, e->B, (e->D != e->A) ? e->D : nullptr, e->Fin, e->dpart,
                   e->d_site_atom, e->d_site_cell, e->d_site_ptr, e->d_atom_site, e->d_lines, e->d_live_ptr,
                   e->d_site_ccell, e->d_linemask, e->coeffs, e->rho, e->Vat,
-                  e->part, e->ovl};
+                  e->part, e->ovl, e->Q, e->Py};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   for (auto ev : e->stage_ev)
@@ -2039,7 +2320,7 @@ int clat_destroy(clat_engine* e) {
   if (e->ggroup) (void)hipGraphExecDestroy(e->ggroup);
   if (e->join_ev) (void)hipEventDestroy(e->join_ev);
   if (e->ov_host) (void)hipHostFree(e->ov_host);
-  if (e->stream) (void)hipStreamDestroy(e->stream);
+  if (e->stream && e->own_stream) (void)hipStreamDestroy(e->stream);
   delete e;
   return CLAT_OK;
 }
@@ -2202,6 +2483,14 @@ int clat_set_atoms(clat_engine* e, const int32_t* cell, int64_t n_atoms, int n_o
     }
     for (int64_t l = 0; l < nlines; ++l) site_ptr[(size_t)l + 1] += site_ptr[(size_t)l];
     for (int64_t a = 0; a < n_atoms; ++a) atom_site[(size_t)a] = last[(size_t)cell[a]];
+    // two-coset path: every site on x = y = z (mod 2) of an even grid, on the
+    // atom path of the shared schedule with every pass on compile-time kernels
+    bool cos = !e->pair_conv && e->shared() && e->xline && e->conv_tiled && e->btile && e->atile &&
+               e->Xi % 2 == 0 && e->Yi % 2 == 0 && e->Zi % 2 == 0 && !site_atom.empty();
+    if (const char* g = std::getenv("CLAT_COSETS")) cos = cos && std::atoi(g) != 0;
+    CosetOrder co;
+    if (cos) cos = coset_order(e, site_atom, site_cell, co);
+    if (cos) cos = ensure_sub(e);
     // occupied x-lines: compact list, CSR over it, compact cells, bit mask
     std::vector<int32_t> lines, live_ptr(1, 0), ccell(site_cell.size());
     // y-pass mask: one word per (z, n1) of the y radix split, bit n2 for y = n1 + R1 n2
@@ -2226,6 +2515,24 @@ int clat_set_atoms(clat_engine* e, const int32_t* cell, int64_t n_atoms, int n_o
     e->tuned_nb = 0;
     e->n_atoms = n_atoms;
     e->n_orb = n_orbitals;
+    if (cos) {
+      // the coarse engine takes the n_u positions as its atoms (one per
+      // coarse cell, so its sites are the positions in order); this engine
+      // keeps the two-coset site order and no fine line structures
+      const int rc = clat_set_atoms(e->sub, co.ucell.data(), (int64_t)co.ucell.size(), n_orbitals);
+      if (rc != CLAT_OK) throw host::Error(rc, std::string("coarse engine: ") + clat_last_error());
+      e->coset = true;
+      e->n_u = (int64_t)co.ucell.size();
+      for (int64_t a = 0; a < n_atoms; ++a) atom_site[(size_t)a] = co.new_of[(size_t)atom_site[(size_t)a]];
+      site_atom.swap(co.site_atom);
+      site_cell.swap(co.site_cell);
+      std::fill(site_ptr.begin(), site_ptr.end(), 0);
+      lines.clear();
+      live_ptr.assign(1, 0);
+      ccell.assign(site_atom.size(), 0);
+      std::fill(mask.begin(), mask.end(), 0u);
+      if (e->initialized) build_coset_tables(e);
+    }
     e->n_sites = (int64_t)site_atom.size();
     e->d_site_atom = (int32_t*)e->dalloc((size_t)e->n_sites * 4);
     e->d_site_cell = (int32_t*)e->dalloc((size_t)e->n_sites * 4);
@@ -2414,7 +2721,7 @@ int clat_reset_kernel_stats(clat_engine* e) {
 int clat_engine_info(clat_engine* e, int dim[3], int64_t* device_bytes) {
   if (!e) return set_err(CLAT_EINVAL, "invalid arguments");
   if (dim) { dim[0] = e->Xi; dim[1] = e->Yi; dim[2] = e->Zi; }
-  if (device_bytes) *device_bytes = e->device_bytes;
+  if (device_bytes) *device_bytes = e->device_bytes + (e->sub ? e->sub->device_bytes : 0);
   return CLAT_OK;
 }
 
@@ -2427,6 +2734,7 @@ int clat_engine_kernels(clat_engine* e, uint32_t* flags) {
   if (e->conv_tiled) f |= CLAT_KERNELS_CONV_TILED;
   if (e->btile) f |= CLAT_KERNELS_B_STRIPS;
   if (e->atile) f |= CLAT_KERNELS_AD_STRIPS;
+  if (e->coset) f |= CLAT_KERNELS_COSETS;
   *flags = f;
   return CLAT_OK;
 }

@@ -410,6 +410,76 @@ __global__ void k_gconv_tiled(const double* __restrict__ G, T* __restrict__ out,
   }
 }
 
+// Kernel tables of the two-coset path from the fine table Gs (the octant
+// (Zi+1)(Yi+1)(Xi+1) of DCT-I(G) / (8 Xi Yi Zi), Xi Yi Zi even).  With the
+// fine grid split into the cosets 2m + p of the coarse grid (Xc = Xi / 2 per
+// axis), the coarse transform of G(2d + p) over the coarse box of Xi points
+// per axis is
+//   Gp(q) = W_2Xi^(-p.q) sum_{s in {0,1}^3} (-1)^(p.s) Gs(q + s Xi)
+// (the fine box split into its eight half-box shifts), already scaled for the
+// coarse inverse.  Gs(k) for k in [Xi, 2Xi) is the octant entry 2Xi - k, so a
+// coarse frequency q reads the octant at q and Xi - q on every axis.
+__device__ __forceinline__ double coarse_sum(const double* __restrict__ Gs, int Xi, int Yi, int Zi, int qx, int qy,
+                                             int qz, bool odd) {
+  const size_t X1 = (size_t)Xi + 1, Y1 = (size_t)Yi + 1;
+  const int xs[2] = {qx, Xi - qx}, ys[2] = {qy, Yi - qy}, zs[2] = {qz, Zi - qz};
+  double t = 0.0;
+#pragma unroll
+  for (int c = 0; c < 2; ++c)
+#pragma unroll
+    for (int b = 0; b < 2; ++b)
+#pragma unroll
+      for (int a = 0; a < 2; ++a) {
+        const double v = Gs[((size_t)zs[c] * Y1 + ys[b]) * X1 + xs[a]];
+        t += (odd && ((a + b + c) & 1)) ? -v : v;
+      }
+  return t;
+}
+
+// p = 0: the kernel within a coset, real and even about Xc on every axis;
+// its octant (Zc+1)(Yc+1)(Xc+1) for the coarse engine's conv table
+__global__ void k_coarse_g0(const double* __restrict__ Gs, int Xi, int Yi, int Zi, double* __restrict__ G0) {
+  const int Xc = Xi / 2, Yc = Yi / 2, Zc = Zi / 2;
+  const size_t n = (size_t)(Xc + 1) * (Yc + 1) * (Zc + 1);
+  for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < n; t += (size_t)gridDim.x * blockDim.x) {
+    const int qx = (int)(t % (Xc + 1));
+    const size_t u = t / (Xc + 1);
+    const int qy = (int)(u % (Yc + 1));
+    const int qz = (int)(u / (Yc + 1));
+    G0[t] = coarse_sum(Gs, Xi, Yi, Zi, qx, qy, qz, false);
+  }
+}
+
+// p = (1,1,1): the kernel from coset 0 to coset 1,
+//   Gt(q) = e^{i pi (qx/Xi + qy/Yi + qz/Zi)} St(q),  St = the alternating sum,
+// odd about Xc on every axis.  Stored tiled for the coupled conv as
+// [ry][column block][fz][c] (fx = block * C + c < W = Xi, ry in [0, Yc], fz in
+// [0, Zi)), holding e^{i pi (fx/Xi + fz/Zi)} St(fx, ry, fz); the y phase and
+// the sign of the y reflection are applied per block (k_colconv2).
+template <typename T>
+__global__ void k_coarse_gt(const double* __restrict__ Gs, int Xi, int Yi, int Zi, int C, int ncb,
+                            C2<T>* __restrict__ Q) {
+  const int Yc = Yi / 2;
+  const size_t n = (size_t)(Yc + 1) * ncb * Zi * C;
+  for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < n; t += (size_t)gridDim.x * blockDim.x) {
+    const int c = (int)(t % C);
+    size_t u = t / C;
+    const int fz = (int)(u % Zi);
+    u /= Zi;
+    const int blk = (int)(u % ncb);
+    const int ry = (int)(u / ncb);
+    const int fx = blk * C + c;
+    C2<T> v = mk<T>((T)0, (T)0);
+    if (fx < Xi) {
+      const double st = coarse_sum(Gs, Xi, Yi, Zi, fx, ry, fz, true);
+      double sn, cs;
+      sincospi((double)fx / Xi + (double)fz / Zi, &sn, &cs);
+      v = mk<T>((T)(st * cs), (T)(st * sn));
+    }
+    Q[t] = v;
+  }
+}
+
 __global__ void k_d2f(const double* __restrict__ a, float* __restrict__ b, size_t n) {
   for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < n; t += (size_t)gridDim.x * blockDim.x)
     b[t] = (float)a[t];
@@ -583,7 +653,8 @@ __global__ void k_coeffs_to_sites(const double2* __restrict__ in, const int32_t*
   const int64_t n = n_sites * n_orb;
   for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x) {
     const int64_t k = t / n_orb;
-    out[t] = in[(int64_t)atom[k] * n_orb + (t - k * n_orb)];
+    const int32_t at = atom[k];  // -1: a two-coset site with no atom in that coset
+    out[t] = at < 0 ? make_double2(0.0, 0.0) : in[(int64_t)at * n_orb + (t - k * n_orb)];
   }
 }
 

@@ -17,7 +17,7 @@
 
 namespace clat {
 
-constexpr int LINE_NB = 8;  // prepares one launch can carry
+constexpr int LINE_NB = 16;  // prepares (or coset planes of prepares) one launch can carry
 
 struct LineArgs {
   int n_live;                   // occupied lines

@@ -21,7 +21,7 @@
 
 namespace clat {
 
-enum PassMode { PM_FWD = 0, PM_CONV = 1, PM_INV = 2 };
+enum PassMode { PM_FWD = 0, PM_CONV = 1, PM_INV = 2, PM_CONV2 = 3 };  // PM_CONV2: coupled two-coset conv (column kernels only)
 
 struct P2Args {
   int L, R1, R2, R1p;

@@ -259,15 +259,21 @@ int clat_engine_info(clat_engine* e, int dim[3], int64_t* device_bytes);
 /* Which kernels the engine's passes run on (bits of *flags): the two-level
  * register FFT kernels (else the LDS Stockham kernels), compile-time column
  * kernels for the y passes, the x line kernels of the atom path (occupied
- * lines only), the tiled conv kernel table, and the column-strip layouts of
- * the intermediate planes.  An introspection hook with no counterpart in the
- * reference: every combination gives the same results. */
+ * lines only), the tiled conv kernel table, the column-strip layouts of the
+ * intermediate planes, and the two-coset path of the atom path (set by
+ * clat_set_atoms when every site lies on x = y = z (mod 2) of an even grid,
+ * as the zincblende and diamond lattices on their a/4 grid do: the
+ * convolution then runs on the two cosets of the coarse grid with half the
+ * points per axis, coupled by a 2 x 2 kernel; CLAT_COSETS=0 in the
+ * environment keeps the single-grid path).  An introspection hook with no
+ * counterpart in the reference: every combination gives the same results. */
 #define CLAT_KERNELS_TWO_LEVEL 1u
 #define CLAT_KERNELS_COLUMN 2u
 #define CLAT_KERNELS_LINE 4u
 #define CLAT_KERNELS_CONV_TILED 8u
 #define CLAT_KERNELS_B_STRIPS 16u
 #define CLAT_KERNELS_AD_STRIPS 32u
+#define CLAT_KERNELS_COSETS 64u
 int clat_engine_kernels(clat_engine* e, uint32_t* flags);
 
 /* ---- planning ---------------------------------------------------------- */
