@@ -1,0 +1,105 @@
+"""The mathematics of the engine's two-coset path, checked on the CPU against
+the oracle (no GPU).
+
+When every site of an even grid lies on x = y = z (mod 2), the sites split
+into the cosets 2m (p = 0) and 2m + (1,1,1) (p = 1) of the coarse grid with
+half the points per axis, and the open-boundary convolution at the sites is
+
+    V_0 = G_0 * rho_0 + G_-t * rho_1,    V_1 = G_t * rho_0 + G_0 * rho_1,
+
+with G_p(d) = G(2d + p) convolved over the coarse grid.  The engine never
+samples G_p: it derives their coarse transforms from the fine DCT-I table Gs
+(the octant of DCT-I(G) / (8XYZ), CoulombCalculator.cpp:57-103) by summing its
+eight half-box shifts (kernels.hpp: k_coarse_g0, k_coarse_gt).  These tests
+pin that derivation against the transforms of the sampled kernels, and the
+whole decomposition against the oracle's eight-round prepare
+(CoulombCalculator.cpp:105-163)."""
+import numpy as np
+import pytest
+
+import oracle as O
+
+
+def coarse_tables(Gs):
+    """Restatement of k_coarse_g0 / k_coarse_gt on the full coarse box:
+    G0(q) = sum_s Gs(q + s X),  Gt(q) = e^{i pi (qx/X + qy/Y + qz/Z)} sum_s (-1)^{|s|} Gs(q + s X),
+    with Gs(k) for k in [X, 2X) read at the octant entry 2X - k."""
+    Z1, Y1, X1 = Gs.shape
+    X, Y, Z = X1 - 1, Y1 - 1, Z1 - 1
+    qz, qy, qx = np.meshgrid(np.arange(Z), np.arange(Y), np.arange(X), indexing="ij")
+    g0 = np.zeros((Z, Y, X))
+    st = np.zeros((Z, Y, X))
+    for c in (0, 1):
+        for b in (0, 1):
+            for a in (0, 1):
+                v = Gs[Z - qz if c else qz, Y - qy if b else qy, X - qx if a else qx]
+                g0 += v
+                st += v * (-1.0) ** (a + b + c)
+    gt = np.exp(1j * np.pi * (qx / X + qy / Y + qz / Z)) * st
+    return g0, gt
+
+
+def sampled_transform(G, p):
+    """Transform of the coarse kernel G_p(d) = G(2d + p), d on the circular
+    coarse box (X points per axis: d in [-X/2, X/2)), scaled by 1 / (8 N_coarse)
+    like the engine's tables; G is the fine real-space octant (evenness gives
+    the other octants)."""
+    Z1, Y1, X1 = G.shape
+    X, Y, Z = X1 - 1, Y1 - 1, Z1 - 1
+    d = [np.where(np.arange(n) < n // 2, np.arange(n), np.arange(n) - n) for n in (Z, Y, X)]
+    dz, dy, dx = np.meshgrid(*d, indexing="ij")
+    K = G[np.abs(2 * dz + p), np.abs(2 * dy + p), np.abs(2 * dx + p)]
+    return np.fft.fftn(K) / (X * Y * Z)   # 8 N_coarse = 8 (X/2)(Y/2)(Z/2)
+
+
+@pytest.mark.parametrize("dims,tf", [((12, 8, 10), None), ((8, 14, 6), None), ((10, 10, 16), 5.65)])
+def test_coset_tables_are_coarse_transforms(dims, tf):
+    X, Y, Z = dims
+    G = O.gtable(X, Y, Z, (0.7, 0.9, 1.1), 0.31, 2.5, tf_lattice=tf)
+    g0, gt = coarse_tables(O.ghat(G))
+    want0 = sampled_transform(G, 0)
+    want1 = sampled_transform(G, 1)
+    scale = np.max(np.abs(want0))
+    assert np.max(np.abs(g0 - want0)) < 1e-13 * scale
+    assert np.max(np.abs(gt - want1)) < 1e-13 * scale
+    # the symmetries the tiled tables rely on: G0 even and St odd about X/2 per axis
+    st = gt * np.exp(-1j * np.pi * np.add.outer(np.add.outer(np.arange(Z) / Z, np.arange(Y) / Y), np.arange(X) / X))
+    assert np.max(np.abs(g0[:, :, 1:] - g0[:, :, :0:-1])) < 1e-13 * scale
+    assert np.max(np.abs(st[:, :, 1:] + st[:, :, :0:-1])) < 1e-13 * scale
+    assert np.max(np.abs(st[:, 1:, :] + st[:, :0:-1, :])) < 1e-13 * scale
+
+
+@pytest.mark.parametrize("dims", [(12, 8, 10), (16, 6, 8)])
+def test_two_coset_convolution_matches_oracle(dims):
+    """Random complex density on x = y = z (mod 2), some sites empty: the
+    coupled coarse convolution equals the oracle's eight-round prepare at every
+    site of both cosets.  A dropped cross term, a conjugate on the wrong side
+    or a coset shift of the wrong sign fails it."""
+    X, Y, Z = dims
+    rng = np.random.default_rng(X * 100 + Y + Z)
+    zz, yy, xx = np.meshgrid(np.arange(Z), np.arange(Y), np.arange(X), indexing="ij")
+    on = ((xx % 2) == (yy % 2)) & ((yy % 2) == (zz % 2)) & (rng.random((Z, Y, X)) < 0.7)
+    F = np.zeros((Z, Y, X), complex)
+    F[on] = rng.standard_normal(on.sum()) + 1j * rng.standard_normal(on.sum())
+    Gs = O.ghat(O.gtable(X, Y, Z, (0.8, 1.0, 1.2), 0.5, 3.0))
+    Vo = O.prepare(F, Gs)
+    g0, gt = coarse_tables(Gs)
+    Xc, Yc, Zc = X // 2, Y // 2, Z // 2
+
+    def fwd(r):
+        P = np.zeros((Z, Y, X), complex)   # coarse box: 2 Xc = X points per axis
+        P[:Zc, :Yc, :Xc] = r
+        return np.fft.fftn(P)
+
+    def inv(P):  # unnormalised inverse (the tables carry the 1 / (8 N_coarse))
+        return np.fft.ifftn(P)[:Zc, :Yc, :Xc] * (X * Y * Z)
+
+    r0, r1 = fwd(F[0::2, 0::2, 0::2]), fwd(F[1::2, 1::2, 1::2])
+    v0 = inv(g0 * r0 + np.conj(gt) * r1)
+    v1 = inv(gt * r0 + g0 * r1)
+    scale = np.max(np.abs(Vo))
+    assert np.max(np.abs(v0 - Vo[0::2, 0::2, 0::2])) < 1e-12 * scale
+    assert np.max(np.abs(v1 - Vo[1::2, 1::2, 1::2])) < 1e-12 * scale
+    # the cross terms carry real weight here: without them coset 1 is far off
+    v1_alone = inv(g0 * r1)
+    assert np.max(np.abs(v1_alone - Vo[1::2, 1::2, 1::2])) > 1e-3 * scale
