@@ -315,11 +315,13 @@ struct clat_engine {
   bool sub_failed = false;      // the coarse engine cannot run this grid
   bool coset = false;           // the current atoms run on the two-coset path
   int64_t n_u = 0;
+  bool fine_pending = false;    // G64 holds the real-space table: its DCT-I and conv table wait for first use
   // kept by the coarse engine of a two-coset path
   clat_engine* kparent = nullptr;  // kernel timing goes to this engine's events
   bool own_stream = true;
   bool pair_conv = false;          // the conv couples planes 2b and 2b + 1
-  void* Q = nullptr;               // cross-coset kernel table (k_coarse_gt)
+  void* Q = nullptr;               // cross-coset kernel table (k_coarse_gt / k_coarse_q)
+  double* Gt64 = nullptr;          // octant of the cross-coset kernel: samples, then DCT-II
   size_t q_elems = 0;
   void* Py = nullptr;              // y phase and reflection sign of Q per fy
   // host buffers of a run, kept with their pages between runs
@@ -1026,9 +1028,12 @@ void fp32x_prepare(clat_engine* e, const double2* si, double2* so) {
   }
 }
 
+void ensure_fine_tables(clat_engine* e);
+
 // dense F -> dense V
 void prepare_dense(clat_engine* e) {
   ensure_dense(e);
+  ensure_fine_tables(e);
   if (e->fast) {
     if (e->rprec == CLAT_PREC_FP64) prepare_fast<double, double, double, double, double, double, double>(e, nullptr, nullptr);
     else if (e->rprec == CLAT_PREC_FP32) prepare_fast<float, float, float, float, float, float, float>(e, nullptr, nullptr);
@@ -1107,29 +1112,81 @@ void dct_axis(clat_engine* e, int axis, double scale) {
   e->kend(K_DCT, 16.0 * (double)e->NG());
 }
 
+// DCT-II (REDFT10) of the lines of e->Gt64 along one axis (M inputs, M + 1
+// outputs per line in the octant's pitch), through the same 2M-point plans
+void dct2_axis(clat_engine* e, int axis, double scale) {
+  const int X1 = e->Xi + 1, Y1 = e->Yi + 1, Z1 = e->Zi + 1;
+  const AxisPlan& p = axis == 0 ? e->dx : (axis == 1 ? e->dy : e->dz);
+  const int L2 = p.L;
+  const int C = std::min(64, floor_pow2(std::max(1, 4096 / L2)));
+  const size_t lds = (size_t)C * L2 * sizeof(double2);
+  e->kbegin(K_DCT);
+  if (axis == 0) {
+    const int ncols = Y1 * Z1;
+    hipLaunchKernelGGL((k_dct2<16, true>), dim3((ncols + 2 * C - 1) / (2 * C)), dim3(NT), lds, e->stream, e->Gt64,
+                       p, e->dtw[0], C, log2i(C), 0LL, 1LL, (long long)X1, ncols, scale);
+  } else if (axis == 1) {
+    const int ncols = X1;
+    hipLaunchKernelGGL((k_dct2<16, false>), dim3(((ncols + 2 * C - 1) / (2 * C)) * Z1), dim3(NT), lds, e->stream,
+                       e->Gt64, p, e->dtw[1], C, log2i(C), (long long)X1 * Y1, (long long)X1, 1LL, ncols, scale);
+  } else {
+    const int ncols = X1 * Y1;
+    hipLaunchKernelGGL((k_dct2<16, false>), dim3((ncols + 2 * C - 1) / (2 * C)), dim3(NT), lds, e->stream,
+                       e->Gt64, p, e->dtw[2], C, log2i(C), 0LL, (long long)X1 * Y1, 1LL, ncols, scale);
+  }
+  LAUNCH_CHECK();
+  e->kend(K_DCT, 16.0 * (double)e->NG());
+}
+
 void tile_conv_table(clat_engine* e);
 
-// kernel tables of the coarse engine of a two-coset path from this engine's
-// fine octant: the within-coset octant (then tiled like any conv table) and
-// the tiled cross-coset table
+// Kernel tables of the coarse engine of a two-coset path.  From the
+// real-space fine table (G64 before its DCT): the within-coset kernel G(2d)
+// through the coarse engine's DCT-I, the cross-coset kernel G(2d + (1,1,1))
+// through a DCT-II, both on the coarse octant (an eighth of the fine one).
+// From a fine table already transformed: the eight half-box shifts of it
+// (k_coarse_g0, k_coarse_gt).  Same tables either way.
 void build_coset_tables(clat_engine* e) {
   clat_engine* s = e->sub;
   s->stream = e->stream;
-  hipLaunchKernelGGL(k_coarse_g0, dim3(grid1d(s->NG())), dim3(256), 0, e->stream, e->G64, e->Xi, e->Yi, e->Zi,
-                     s->G64);
-  LAUNCH_CHECK();
-  tile_conv_table(s);
-  if (s->Gc32)
-    hipLaunchKernelGGL(k_coarse_gt<float>, dim3(grid1d(s->q_elems)), dim3(256), 0, e->stream, e->G64, e->Xi, e->Yi,
-                       e->Zi, s->gc_c, s->gc_ncb, (float2*)s->Q);
-  else
-    hipLaunchKernelGGL(k_coarse_gt<double>, dim3(grid1d(s->q_elems)), dim3(256), 0, e->stream, e->G64, e->Xi,
-                       e->Yi, e->Zi, s->gc_c, s->gc_ncb, (double2*)s->Q);
-  LAUNCH_CHECK();
+  if (e->fine_pending) {
+    hipLaunchKernelGGL(k_coset_samples, dim3(grid1d(s->NG())), dim3(256), 0, e->stream, e->G64, e->Xi, e->Yi,
+                       e->Zi, s->G64, s->Gt64);
+    LAUNCH_CHECK();
+    const double sc = 1.0 / (8.0 * (double)s->Xi * (double)s->Yi * (double)s->Zi);
+    dct_axis(s, 0, 1.0);
+    dct_axis(s, 1, 1.0);
+    dct_axis(s, 2, sc);
+    dct2_axis(s, 0, 1.0);
+    dct2_axis(s, 1, 1.0);
+    dct2_axis(s, 2, sc);
+    tile_conv_table(s);
+    if (s->Gc32)
+      hipLaunchKernelGGL(k_coarse_q<float>, dim3(grid1d(s->q_elems)), dim3(256), 0, e->stream, s->Gt64, s->Xi,
+                         s->Yi, s->Zi, s->gc_c, s->gc_ncb, (float2*)s->Q);
+    else
+      hipLaunchKernelGGL(k_coarse_q<double>, dim3(grid1d(s->q_elems)), dim3(256), 0, e->stream, s->Gt64, s->Xi,
+                         s->Yi, s->Zi, s->gc_c, s->gc_ncb, (double2*)s->Q);
+    LAUNCH_CHECK();
+  } else {
+    hipLaunchKernelGGL(k_coarse_g0, dim3(grid1d(s->NG())), dim3(256), 0, e->stream, e->G64, e->Xi, e->Yi, e->Zi,
+                       s->G64);
+    LAUNCH_CHECK();
+    tile_conv_table(s);
+    if (s->Gc32)
+      hipLaunchKernelGGL(k_coarse_gt<float>, dim3(grid1d(s->q_elems)), dim3(256), 0, e->stream, e->G64, e->Xi,
+                         e->Yi, e->Zi, s->gc_c, s->gc_ncb, (float2*)s->Q);
+    else
+      hipLaunchKernelGGL(k_coarse_gt<double>, dim3(grid1d(s->q_elems)), dim3(256), 0, e->stream, e->G64, e->Xi,
+                         e->Yi, e->Zi, s->gc_c, s->gc_ncb, (double2*)s->Q);
+    LAUNCH_CHECK();
+  }
   s->initialized = true;
 }
 
-void finish_interaction(clat_engine* e) {
+// the fine table's DCT-I (and FP32 copy) and conv table, once per interaction
+void ensure_fine_tables(clat_engine* e) {
+  if (!e->fine_pending) return;
   dct_axis(e, 0, 1.0);
   dct_axis(e, 1, 1.0);
   dct_axis(e, 2, 1.0 / (8.0 * (double)e->Xi * (double)e->Yi * (double)e->Zi));
@@ -1138,7 +1195,15 @@ void finish_interaction(clat_engine* e) {
     LAUNCH_CHECK();
   }
   tile_conv_table(e);
+  e->fine_pending = false;
+}
+
+// G64 holds a new real-space table: on the two-coset path only the coarse
+// tables are built (the fine ones wait for a single-grid prepare)
+void finish_interaction(clat_engine* e) {
+  e->fine_pending = true;
   if (e->coset) build_coset_tables(e);
+  else ensure_fine_tables(e);
   e->initialized = true;
   e->prepared = false;
 }
@@ -1486,6 +1551,16 @@ std::vector<int64_t> tune_key(const clat_engine* e) {
           e->atile ? 1 : 0, e->btile ? 1 : 0, e->n_cu, e->coset ? e->sub->n_live : -1};
 }
 
+// The fixed rule's prepares per launch: the engine's own, or on the
+// two-coset path eight where sixteen coarse B planes take at most 1.1 GB
+// (the 1M-atom grid's coset planes: 2.88 ms per prepare stage with eight
+// against 3.00 with four)
+int batch_rule(const clat_engine* e) {
+  if (!e->coset || std::getenv("CLAT_BATCH")) return e->nb_rule;
+  const size_t planes = 16 * e->sub->b_set * e->sub->s_b;
+  return std::min(e->nb_max, planes <= ((size_t)1100 << 20) ? 8 : 4);
+}
+
 // Prepares per pass launch for this run (the autotuner, the counterpart of
 // the reference's FFTW_MEASURE plans, CoulombCalculator.cpp:23,31).  Batches
 // of 1, 2, 4 and 8 prepares run the same kernels on the same data, so they
@@ -1495,7 +1570,7 @@ std::vector<int64_t> tune_key(const clat_engine* e) {
 // rule's batch), and the fastest is kept for the engine and for the process.
 int choose_batch(clat_engine* e, const Units& units, const std::vector<int>& slots) {
   if (!(e->xline && e->shared())) return 1;  // batching lives on the line-kernel path only
-  if (!e->tune) return e->nb_rule;
+  if (!e->tune) return batch_rule(e);
   if (e->tuned_nb) return e->tuned_nb;
   const std::vector<int64_t> key = tune_key(e);
   {
@@ -1514,7 +1589,7 @@ int choose_batch(clat_engine* e, const Units& units, const std::vector<int>& slo
   e->kmask = 0;  // the trials stay out of the kernel statistics
   std::vector<double> best(cand.size(), 1e30);
   try {
-    run_prepares(e, units, slots, e->nb_rule);  // warm-up: code objects loaded, caches filled
+    run_prepares(e, units, slots, batch_rule(e));  // warm-up: code objects loaded, caches filled
     for (int round = 0; round < 2; ++round) {
       const int nc = (int)cand.size();
       HIPX(hipEventRecord(e->tune_ev[0], e->stream));
@@ -1912,6 +1987,7 @@ bool ensure_sub(clat_engine* e) {
     // Py[fy] = sign * e^{i pi fy / (2 Yc)}, sign -1 above fy = Yc
     s->q_elems = (size_t)(s->Yi + 1) * s->gc_ncb * (2 * s->Zi) * s->gc_c;  // rows fz in [0, 2 Zc)
     s->Q = s->dalloc(s->q_elems * csz);
+    s->Gt64 = (double*)s->dalloc(s->NG() * sizeof(double));
     const int ny = 2 * s->Yi;
     std::vector<double> py((size_t)2 * ny);
     for (int fy = 0; fy < ny; ++fy) {
@@ -2307,7 +2383,7 @@ int clat_destroy(clat_engine* e) {
                   e->G32, e->Gc64, e->Gc32, e->F, e->V, e->A, e->B, (e->D != e->A) ? e->D : nullptr, e->Fin, e->dpart,
                   e->d_site_atom, e->d_site_cell, e->d_site_ptr, e->d_atom_site, e->d_lines, e->d_live_ptr,
                   e->d_site_ccell, e->d_linemask, e->coeffs, e->rho, e->Vat,
-                  e->part, e->ovl, e->Q, e->Py};
+                  e->part, e->ovl, e->Q, e->Py, e->Gt64};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   for (auto ev : e->stage_ev)
@@ -2532,6 +2608,8 @@ int clat_set_atoms(clat_engine* e, const int32_t* cell, int64_t n_atoms, int n_o
       ccell.assign(site_atom.size(), 0);
       std::fill(mask.begin(), mask.end(), 0u);
       if (e->initialized) build_coset_tables(e);
+    } else if (e->initialized) {
+      ensure_fine_tables(e);
     }
     e->n_sites = (int64_t)site_atom.size();
     e->d_site_atom = (int32_t*)e->dalloc((size_t)e->n_sites * 4);
@@ -2748,7 +2826,7 @@ int clat_set_tuning(clat_engine* e, int measure) {
 int clat_tuning_info(clat_engine* e, int* batch, double trial_ms[4], int64_t* graph_replays,
                      int64_t* graph_captures) {
   if (!e) return set_err(CLAT_EINVAL, "invalid arguments");
-  if (batch) *batch = e->tune ? e->tuned_nb : e->nb_rule;
+  if (batch) *batch = e->tune ? e->tuned_nb : batch_rule(e);
   if (trial_ms)
     for (int i = 0; i < 4; ++i) trial_ms[i] = e->tune_ms[i];
   if (graph_replays) *graph_replays = e->graph_replays;

@@ -358,6 +358,108 @@ __global__ __launch_bounds__(NT) void k_dct1(double* __restrict__ G, AxisPlan p,
   (void)nlines;
 }
 
+// DCT-II (FFTW REDFT10, unnormalised) of every line along one axis,
+//   Y_k = 2 sum_{n=0}^{M-1} x_n cos(pi (2n + 1) k / (2M)),  k = 0..M (Y_M = 0),
+// through the 2M-point FFT of the zero-padded line:
+//   Y_k = 2 Re(e^{-i pi k / (2M)} X_k).
+// Two real columns c and c + C share one complex line (the real and
+// imaginary parts); their transforms are separated from X_k and X_{2M-k}.
+// Line layout and launch shape as k_dct1; a line holds M inputs and M + 1
+// outputs in place (the table's pitch has room for the extra one).
+template <int EMAX, bool LM>
+__global__ __launch_bounds__(NT) void k_dct2(double* __restrict__ G, AxisPlan p,
+                                             const double2* __restrict__ tw, int C, int lgC,
+                                             long long os, long long is, long long cs, int ncols,
+                                             double scale) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  double2* buf = reinterpret_cast<double2*>(smem_raw);
+  const int L2 = p.L;  // 2M
+  const int M = L2 / 2;
+  const int ncb = (ncols + 2 * C - 1) / (2 * C);
+  const int o = blockIdx.x / ncb;
+  const int c0 = (blockIdx.x - o * ncb) * 2 * C;
+  const int nC = min(2 * C, ncols - c0);
+  double* base = G + (size_t)o * os + (size_t)c0 * cs;
+  const int tot = C * L2;
+  for (int e = threadIdx.x; e < tot; e += NT) {
+    int c, m;
+    if (LM) { c = (int)fdiv((uint32_t)e, p.l_magic); m = e - c * L2; }
+    else { c = e & (C - 1); m = e >> lgC; }
+    double v0 = 0.0, v1 = 0.0;
+    if (m < M) {
+      if (c < nC) v0 = base[(size_t)m * is + (size_t)c * cs];
+      if (c + C < nC) v1 = base[(size_t)m * is + (size_t)(c + C) * cs];
+    }
+    buf[LM ? c * L2 + m : m * C + c] = make_double2(v0, v1);
+  }
+  __syncthreads();
+  lds_fft<false, EMAX, LM, double>(buf, tw, p, C, lgC);
+  const int tot1 = C * (M + 1);
+  for (int e = threadIdx.x; e < tot1; e += NT) {
+    int c, k;
+    if (LM) { c = e / (M + 1); k = e - c * (M + 1); }
+    else { c = e & (C - 1); k = e >> lgC; }
+    const int kr = k == 0 ? 0 : L2 - k;
+    const double2 x = buf[LM ? c * L2 + k : k * C + c];
+    const double2 r = buf[LM ? c * L2 + kr : kr * C + c];
+    double sn, cn;
+    sincospi((double)k / (double)L2, &sn, &cn);
+    const double ya = cn * (x.x + r.x) + sn * (x.y - r.y);
+    const double yb = cn * (x.y + r.y) - sn * (x.x - r.x);
+    if (c < nC) base[(size_t)k * is + (size_t)c * cs] = ya * scale;
+    if (c + C < nC) base[(size_t)k * is + (size_t)(c + C) * cs] = yb * scale;
+  }
+}
+
+// Real-space samples of the two-coset kernels from the fine real-space octant
+// G ((Zi+1)(Yi+1)(Xi+1), G at fine offset (ix, iy, iz)): G0 = G(2d) on the
+// coarse octant d in [0, Xc]^3, Gt = G(2d + (1,1,1)) on d in [0, Xc)^3 in a
+// table of the same shape (zero where d reaches Xc on some axis).
+__global__ void k_coset_samples(const double* __restrict__ G, int Xi, int Yi, int Zi, double* __restrict__ G0,
+                                double* __restrict__ Gt) {
+  const int Xc = Xi / 2, Yc = Yi / 2, Zc = Zi / 2;
+  const size_t X1 = (size_t)Xi + 1, Y1 = (size_t)Yi + 1;
+  const size_t n = (size_t)(Xc + 1) * (Yc + 1) * (Zc + 1);
+  for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < n; t += (size_t)gridDim.x * blockDim.x) {
+    const int dx = (int)(t % (Xc + 1));
+    const size_t u = t / (Xc + 1);
+    const int dy = (int)(u % (Yc + 1));
+    const int dz = (int)(u / (Yc + 1));
+    G0[t] = G[((size_t)(2 * dz) * Y1 + 2 * dy) * X1 + 2 * dx];
+    Gt[t] = (dx < Xc && dy < Yc && dz < Zc) ? G[((size_t)(2 * dz + 1) * Y1 + 2 * dy + 1) * X1 + 2 * dx + 1] : 0.0;
+  }
+}
+
+// The tiled cross-coset table of k_colconv2 from the octant St (Zc+1)(Yc+1)(Xc+1)
+// of DCT-II(Gt) / (8 Xc Yc Zc): for fx, fz in [0, 2Xc), [0, 2Zc) and ry in [0, Yc],
+//   Q = e^{i pi (fx / 2Xc + fz / 2Zc)} s(fx) s(fz) St[r(fz)][ry][r(fx)],
+// r(f) = f up to Xc, else 2Xc - f, with s(f) = -1 past Xc (St is odd about Xc).
+template <typename T>
+__global__ void k_coarse_q(const double* __restrict__ St, int Xc, int Yc, int Zc, int C, int ncb,
+                           C2<T>* __restrict__ Q) {
+  const int W = 2 * Xc, Z2 = 2 * Zc;
+  const size_t n = (size_t)(Yc + 1) * ncb * Z2 * C;
+  for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < n; t += (size_t)gridDim.x * blockDim.x) {
+    const int c = (int)(t % C);
+    size_t u = t / C;
+    const int fz = (int)(u % Z2);
+    u /= Z2;
+    const int blk = (int)(u % ncb);
+    const int ry = (int)(u / ncb);
+    const int fx = blk * C + c;
+    C2<T> v = mk<T>((T)0, (T)0);
+    if (fx < W) {
+      const int rx = fx <= Xc ? fx : W - fx, rz = fz <= Zc ? fz : Z2 - fz;
+      const double sg = ((fx > Xc) != (fz > Zc)) ? -1.0 : 1.0;
+      const double st = sg * St[((size_t)rz * (Yc + 1) + ry) * (Xc + 1) + rx];
+      double sn, cn;
+      sincospi((double)fx / W + (double)fz / Z2, &sn, &cn);
+      v = mk<T>((T)(st * cn), (T)(st * sn));
+    }
+    Q[t] = v;
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Small elementwise kernels
 // ---------------------------------------------------------------------------

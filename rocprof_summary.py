@@ -35,19 +35,21 @@ def kind_of(name):
     m = re.search(r"k_colpass<(\w+), (\w+), (\w+), (\d),", name)
     if m:
         return {0: "yfwd", 1: "zconv", 2: "yinv"}[int(m.group(4))]
+    if "k_colconv2<" in name:
+        return "zconv"
     m = re.search(r"k_linepass<(\w+), (\w+), (\w+), (\d),", name)
     if m:
         return {0: "xfwd", 2: "xinv"}[int(m.group(4))]
     for key in ("k_products_direct", "k_products", "k_overlap_partial", "k_overlap_reduce", "k_dct1",
                 "k_gconv", "k_gtable", "k_gpad", "k_d2f", "k_xfwd", "k_xinv", "k_col", "k_grid_dot",
-                "k_atoms_from_sites", "copyBuffer"):
+                "k_atoms_from_sites", "k_coarse_g0", "k_coarse_gt", "copyBuffer"):
         if key in name:
             return key[2:] if key.startswith("k_") else key
     return name.split("(")[0][:60]
 
 
 def short(name):
-    m = re.search(r"(k_(?:pass2|colpass|linepass)<[^>]*>)", name)
+    m = re.search(r"(k_(?:pass2|colpass|linepass|colconv2)<[^>]*>)", name)
     return m.group(1) if m else name.split("(")[0].replace("void ", "")[:70]
 
 

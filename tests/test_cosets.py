@@ -7,13 +7,14 @@ half the points per axis, and the open-boundary convolution at the sites is
 
     V_0 = G_0 * rho_0 + G_-t * rho_1,    V_1 = G_t * rho_0 + G_0 * rho_1,
 
-with G_p(d) = G(2d + p) convolved over the coarse grid.  The engine never
-samples G_p: it derives their coarse transforms from the fine DCT-I table Gs
-(the octant of DCT-I(G) / (8XYZ), CoulombCalculator.cpp:57-103) by summing its
-eight half-box shifts (kernels.hpp: k_coarse_g0, k_coarse_gt).  These tests
-pin that derivation against the transforms of the sampled kernels, and the
-whole decomposition against the oracle's eight-round prepare
-(CoulombCalculator.cpp:105-163)."""
+with G_p(d) = G(2d + p) convolved over the coarse grid.  The engine builds the
+coarse transforms of G_0 and G_(1,1,1) in one of two ways: from the fine DCT-I
+table Gs (the octant of DCT-I(G) / (8XYZ), CoulombCalculator.cpp:57-103) by
+summing its eight half-box shifts (kernels.hpp: k_coarse_g0, k_coarse_gt), or
+from the real-space samples through a coarse DCT-I and DCT-II (k_coset_samples,
+k_dct2, k_coarse_q).  These tests pin both against the transforms of the
+sampled kernels, and the whole decomposition against the oracle's eight-round
+prepare (CoulombCalculator.cpp:105-163)."""
 import numpy as np
 import pytest
 
@@ -67,6 +68,28 @@ def test_coset_tables_are_coarse_transforms(dims, tf):
     assert np.max(np.abs(g0[:, :, 1:] - g0[:, :, :0:-1])) < 1e-13 * scale
     assert np.max(np.abs(st[:, :, 1:] + st[:, :, :0:-1])) < 1e-13 * scale
     assert np.max(np.abs(st[:, 1:, :] + st[:, :0:-1, :])) < 1e-13 * scale
+
+
+@pytest.mark.parametrize("dims", [(12, 8, 10), (8, 14, 6)])
+def test_coset_tables_from_real_space_samples(dims):
+    """The second route to the same tables, from the real-space fine table
+    before its DCT (kernels.hpp: k_coset_samples, k_dct2, k_coarse_q): the
+    within-coset octant is DCT-I(G(2d)) / (8 N_coarse) on d in [0, X/2]^3, and
+    the cross-coset one e^{i pi q.(1/X)} DCT-II(G(2d + 1)) / (8 N_coarse) on d in
+    [0, X/2)^3, odd about X/2."""
+    import scipy.fft as sf
+    X, Y, Z = dims
+    Xc, Yc, Zc = X // 2, Y // 2, Z // 2
+    G = O.gtable(X, Y, Z, (0.75, 0.95, 1.05), 0.2, 4.0)
+    g0, gt = coarse_tables(O.ghat(G))
+    n8 = 8.0 * Xc * Yc * Zc
+    g0_route = O.dct1_3d(G[0::2, 0::2, 0::2]) / n8
+    st_route = sf.dctn(G[1:Z:2, 1:Y:2, 1:X:2], type=2) / n8
+    qz, qy, qx = np.meshgrid(np.arange(Zc), np.arange(Yc), np.arange(Xc), indexing="ij")
+    gt_route = np.exp(1j * np.pi * (qx / X + qy / Y + qz / Z)) * st_route
+    scale = np.max(np.abs(g0_route))
+    assert np.max(np.abs(g0[:Zc + 1, :Yc + 1, :Xc + 1] - g0_route)) < 1e-13 * scale
+    assert np.max(np.abs(gt[:Zc, :Yc, :Xc] - gt_route)) < 1e-13 * scale
 
 
 @pytest.mark.parametrize("dims", [(12, 8, 10), (16, 6, 8)])

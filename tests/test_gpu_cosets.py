@@ -149,3 +149,38 @@ def test_cosets_batches_bit_identical(monkeypatch):
         if ref is None:
             ref = runs[0]
         assert np.array_equal(runs[0], ref), batch
+
+
+@pytest.mark.parametrize("prec", [clat.PREC_FP64, clat.PREC_FP32X], ids=["fp64", "fp32x"])
+def test_coset_table_routes_agree(prec, monkeypatch):
+    """The coarse tables built from the transformed fine table (interaction
+    set before the atoms: the eight half-box shifts) and from the real-space
+    samples (interaction set again with the atoms in place: coarse DCT-I and
+    DCT-II) give the same integrals.  The dense protocol (set_input, prepare,
+    calculate) on the same engine afterwards builds the fine table it needs."""
+    nh, ne = 2, 1
+    w = zb.workload(16, 16, 16, nh, ne, config=6, sigma_frac=0.2, n_orb=4)
+    quads = clat.pattern_enumerate("****", nh + ne, nh)
+    eng = _engine(w, w["cells"], prec, 4, w["states"], True, monkeypatch)
+    assert "cosets" in eng.kernels()
+    shifted = eng.compute_integrals(quads)
+    eng.initialize(clat.InteractionSimple(w["step"], 0.0, 10.89))
+    sampled = eng.compute_integrals(quads)
+    X, Y, Z = w["dims"]
+    F = np.zeros((Z, Y, X), np.complex128)
+    F[3, 5, 7], F[40, 22, 9], F[61, 60, 2] = 1.0 + 0.5j, -0.7, 0.25j
+    clat._check(clat.lib().clat_set_input(eng.handle, F.view(np.float64).reshape(-1)))
+    clat._check(clat.lib().clat_prepare(eng.handle))
+    out = np.zeros(2)
+    clat._check(clat.lib().clat_calculate(eng.handle, out))
+    eng.close()
+    Gh = O.ghat(O.gtable(X, Y, Z, w["step"], 0.0, 10.89))
+    dense_want = O.calculate(O.prepare(F, Gh), F)
+    assert abs(complex(out[0], out[1]) - dense_want) < (1e-12 if prec == clat.PREC_FP64 else 1e-5) * abs(dense_want)
+    want = _oracle(w, w["cells"], w["states"], quads, nh + ne)
+    if prec == clat.PREC_FP64:
+        assert np.max(rel_err(sampled, shifted)) < 1e-12
+        check(sampled, want, prec, quads)
+    else:
+        assert np.max(rel_err(sampled, shifted)) < 1e-5
+        check_fp32x_model(sampled, want, quads, "sampled tables fp32x")
