@@ -411,18 +411,15 @@ __global__ __launch_bounds__((col_threads<C, R1>())) void k_colpass(const C2<TI>
 // 2 d + t_i - t_j.  Per frequency the two transformed columns are mixed by
 //   [V0; V1] = [g0, conj(gt); gt, g0] [R0; R1]
 // with g0 = the kernel within a coset (real, the conv-table tile of the
-// single-plane pass) and gt = the kernel from coset 0 to coset 1,
-//   gt(fx, fy, fz) = Q[r(fy)][column block][fz][c] * Py[fy]
-// (Q holds the x and z phases, Py the y phase and the sign of the y
-// reflection).  Same blocks, index maps, strips and batching as the conv
-// mode of k_colpass; the second coset's plane is a.cset elements after the
-// first, in and out.
+// single-plane pass, even in fy and fz) and gt = the kernel from coset 0 to
+// coset 1 (complex, a tile per fy: Q[fy][column block][fz][c]).  Same
+// blocks, index maps, strips and batching as the conv mode of k_colpass; the
+// second coset's plane is a.cset elements after the first, in and out.
 template <typename TI, typename TC, typename TO, int R1, int R2, int C>
 __global__ __launch_bounds__((col_threads<C, R1>())) void k_colconv2(const C2<TI>* in, C2<TO>* out, ColArgs2 a,
                                                                    const C2<TC>* __restrict__ T2L,
                                                                    const TC* __restrict__ G,
-                                                                   const C2<TC>* __restrict__ Q,
-                                                                   const C2<TC>* __restrict__ Py) {
+                                                                   const C2<TC>* __restrict__ Q) {
   using CT = C2<TC>;
   constexpr int L = R1 * R2, NSUB = 2;
   constexpr uint32_t SI = sizeof(C2<TI>), SO = sizeof(C2<TO>), SC = sizeof(CT);
@@ -485,9 +482,8 @@ __global__ __launch_bounds__((col_threads<C, R1>())) void k_colconv2(const C2<TI
   const int fy = o * a.gym + a.gya;
   const int ry = min(fy, a.Y2 - fy);
   const TC* Gb = G + ((size_t)(ry * a.npar + a.par) * a.ncb + cbk) * a.GZ1 * C;
-  const CT* Qb = Q + ((size_t)ry * a.ncb + cbk) * (size_t)(2 * L) * C;
+  const CT* Qb = Q + ((size_t)fy * a.ncb + cbk) * (size_t)(2 * L) * C;
   const uint32_t gcol = (uint32_t)c * (uint32_t)sizeof(TC), qcol = (uint32_t)c * SC;
-  const CT py = Py[fy];
 
 #pragma unroll
   for (int s = 0; s < NSUB; ++s) {
@@ -544,7 +540,7 @@ __global__ __launch_bounds__((col_threads<C, R1>())) void k_colconv2(const C2<TI
           off = (uint32_t)min(fz, 2 * L - fz) * RB;
         }
         const TC g = ldg_off(Gb, off + gcol);
-        const CT gt = cmul<TC>(ldg_off(Qb, (uint32_t)(2 * R2 * k1 + q) * (C * SC) + qcol), py);
+        const CT gt = ldg_off(Qb, (uint32_t)(2 * R2 * k1 + q) * (C * SC) + qcol);
         const CT a0 = v[k1], a1 = w[k1];
         v[k1] = mk<TC>(g * a0.x + gt.x * a1.x + gt.y * a1.y, g * a0.y + gt.x * a1.y - gt.y * a1.x);
         w[k1] = mk<TC>(gt.x * a0.x - gt.y * a0.y + g * a1.x, gt.x * a0.y + gt.y * a0.x + g * a1.y);
@@ -619,8 +615,7 @@ struct ColLaunch {
   ColArgs2 a;
   const void* T2L;
   const void* G;
-  const void* Q;   // coupled conv: the cross-coset table and the y factors
-  const void* Py;
+  const void* Q;   // coupled conv: the cross-coset table
 };
 
 template <int R1, int R2, typename TC>
@@ -653,8 +648,7 @@ hipError_t conv2_go(const ColLaunch& L) {
     return hipErrorInvalidValue;  // the host keeps such grids on the single-plane path
   } else {
     hipLaunchKernelGGL((k_colconv2<TI, TC, TO, R1, R2, C>), L.grid, dim3(nth), lds, L.stream, (const C2<TI>*)L.in,
-                       (C2<TO>*)L.out, L.a, (const C2<TC>*)L.T2L, (const TC*)L.G, (const C2<TC>*)L.Q,
-                       (const C2<TC>*)L.Py);
+                       (C2<TO>*)L.out, L.a, (const C2<TC>*)L.T2L, (const TC*)L.G, (const C2<TC>*)L.Q);
     return hipGetLastError();
   }
 }

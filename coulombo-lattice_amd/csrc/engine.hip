@@ -320,10 +320,15 @@ struct clat_engine {
   clat_engine* kparent = nullptr;  // kernel timing goes to this engine's events
   bool own_stream = true;
   bool pair_conv = false;          // the conv couples planes 2b and 2b + 1
-  void* Q = nullptr;               // cross-coset kernel table (k_coarse_gt / k_coarse_q)
+  void* Q = nullptr;               // cross-coset kernel table (k_coarse_q / k_coarse_q_half)
+  size_t q_cap = 0;                // its capacity in elements
   double* Gt64 = nullptr;          // octant of the cross-coset kernel: samples, then DCT-II
+  // half-box form (every coarse site on mx + my + mz even): x frequencies
+  // f < fmax carried in wcols columns (the strip width's multiple), kernels
+  // summed over the half-box shift; otherwise wcols = fmax = 2 Xi
+  bool half = false;
+  int wcols = 0, fmax = 0;
   size_t q_elems = 0;
-  void* Py = nullptr;              // y phase and reflection sign of Q per fy
   // host buffers of a run, kept with their pages between runs
   std::vector<host::PlanStep> plan_steps;
   std::vector<char> plan_in_cover, plan_leads;
@@ -519,7 +524,7 @@ bool launch_col(clat_engine* e, const PassCfg& pc, const P2Args& a, int n_outer,
   c.cset = 0;
   L.T2L = sizeof(TC) == 8 ? (const void*)e->dtw[axis] : (const void*)e->dtw32[axis];
   L.G = gc_of<TC>(e);
-  L.Q = L.Py = nullptr;
+  L.Q = nullptr;
   constexpr int combo = combo_of<TI, TC, TO>();
   e->kbegin(kind);
   hipError_t err = hipErrorInvalidValue;
@@ -541,7 +546,7 @@ bool launch_col(clat_engine* e, const PassCfg& pc, const P2Args& a, int n_outer,
 // into blockIdx.x).
 template <typename TB, typename TC>
 void launch_conv2(clat_engine* e, const PassCfg& pc, const P2Args& a, int n_outer, double bytes) {
-  if (!e->Q || !e->Py || (sizeof(TC) == 4) != (e->Gc32 != nullptr))
+  if (!e->Q || (sizeof(TC) == 4) != (e->Gc32 != nullptr))
     throw host::Error(CLAT_EDEVICE, "coupled conv without its kernel tables");
   const int C = col_c<TC>(pc.R1 * pc.R2);
   ColLaunch L;
@@ -577,7 +582,6 @@ void launch_conv2(clat_engine* e, const PassCfg& pc, const P2Args& a, int n_oute
   L.T2L = sizeof(TC) == 8 ? (const void*)e->dtw[2] : (const void*)e->dtw32[2];
   L.G = gc_of<TC>(e);
   L.Q = e->Q;
-  L.Py = e->Py;
   constexpr int combo = combo_of<TB, TC, TB>();
   e->kbegin(K_ZCONV);
   hipError_t err = hipErrorInvalidValue;
@@ -658,9 +662,10 @@ void launch_line(clat_engine* e, const void* in, void* out, long long bs, int ki
   }
   if ((double)e->a_set * (double)std::max(e->s_a, e->s_d) >= (double)COL_RANGE - 4096.0)
     throw host::Error(CLAT_EUNSUP, "grid too large for the line kernels");
-  const int W = 2 * e->Xi;
+  const int W = e->wcols;
   a.Yl = e->Yi;
   a.lz = e->Yi * W;
+  a.fmax = e->fmax;
   if (e->atile) {
     a.ly = e->gc_c;
     a.lgsw = log2i(e->gc_c);
@@ -748,7 +753,7 @@ void prepare_fast(clat_engine* e, const double2* sites_in, double2* sites_out) {
     a.out_os = (long long)nsub * Xi; a.out_is = 1;
     a.rphase = nsub == 1 ? 1 : 0;
     a.kx = kx; a.ky = ky; a.kz = kz;
-    const double bytes = (atoms ? site_bytes : si * N) + sa * N * nsub * live;
+    const double bytes = (atoms ? site_bytes : si * N) + sa * (nsub == 2 ? (double)e->a_set : N) * live;
     if (lk) {
       launch_line<double, TC, TA, PM_FWD>(e, nullptr, out, (long long)e->a_set, K_XFWD, bytes * nb);
     } else if (atoms) {
@@ -772,7 +777,7 @@ void prepare_fast(clat_engine* e, const double2* sites_in, double2* sites_out) {
     a.acc = acc;
     if (lk) {
       launch_line<TD, TCX, double, PM_INV>(e, in, nullptr, (long long)e->d_set, K_XINV,
-                                           (sd * N * nsub * live + site_bytes) * nb);
+                                           (sd * (double)e->d_set * live + site_bytes) * nb);
     } else if (atoms) {
       a.site_ptr = e->d_site_ptr;
       a.site_cell = e->d_site_cell;
@@ -786,7 +791,7 @@ void prepare_fast(clat_engine* e, const double2* sites_in, double2* sites_out) {
   };
 
   if (e->shared()) {
-    const long long W = 2LL * Xi;
+    const long long W = e->wcols;  // x-frequency columns carried (2 Xi, or the half box)
     // B holds the y-forward output as [fy][z][fx] (the z pass reads rows W
     // apart) or, with e->btile, as [fy][fx / C][z][fx % C]: element (fy, z)
     // of column strip j at fy Z W + j Z C + z C
@@ -814,7 +819,7 @@ void prepare_fast(clat_engine* e, const double2* sites_in, double2* sites_out) {
       if (at) { a.in_is = Cb; a.in_cbs = (long long)Yi * Cb; a.outer_x = 1; }
       if (lk) { a.rowmask = e->d_linemask; a.mask_os = e->cyf.R1; batch(a, (long long)e->a_set, (long long)e->b_set); }
       launch2<TA, TC, TB, PM_FWD, false>(e, e->cyf, a, Zi, 1, e->A, e->B, K_YFWD,
-                                         (sa * 2 * live + sb * 4) * N * nb);
+                                         (sa * (double)e->a_set * live + sb * (double)e->b_set) * nb);
     }
     {
       P2Args a = base_args(e);
@@ -832,7 +837,7 @@ void prepare_fast(clat_engine* e, const double2* sites_in, double2* sites_out) {
         a.nb = nb / 2;
         a.in_bs = a.out_bs = 2LL * (long long)e->b_set;
         launch_conv2<TB, TCZ>(e, e->czc, a, 2 * Yi,
-                              sb * 8 * N * nb + (double)sizeof(TCZ) * (double)e->NG() +
+                              sb * 2 * (double)e->b_set * nb + (double)sizeof(TCZ) * (double)e->gc_elems +
                                   (double)sizeof(C2<TCZ>) * (double)e->q_elems);
       } else {
         launch2<TB, TCZ, TB, PM_CONV, false>(e, e->czc, a, 2 * Yi, 2, e->B, e->B, K_ZCONV,
@@ -848,7 +853,7 @@ void prepare_fast(clat_engine* e, const double2* sites_in, double2* sites_out) {
       a.outer_x = yo;
       if (lk) { a.rowmask = e->d_linemask; a.mask_os = e->cyi.R1; batch(a, (long long)e->b_set, (long long)e->d_set); }
       launch2<TB, TCY, TD, PM_INV, false>(e, e->cyi, a, Zi, 1, e->B, e->D, K_YINV,
-                                          (sb * 4 + sd * 2 * live) * N * nb);
+                                          (sb * (double)e->b_set + sd * (double)e->d_set * live) * nb);
     }
     xinv(2, 0, 0, 0, 0, e->D);
   } else {
@@ -956,7 +961,7 @@ void prepare_generic(clat_engine* e) {
   base.GY1 = Yi + 1;
 
   if (e->shared()) {
-    const long long W = 2LL * Xi;
+    const long long W = e->wcols;  // x-frequency columns carried (2 Xi, or the half box)
     xfwd(2, 0, 0, 0, A);
     ColArgs a = base;
     a.W = (int)W;
@@ -1145,7 +1150,9 @@ void tile_conv_table(clat_engine* e);
 // through the coarse engine's DCT-I, the cross-coset kernel G(2d + (1,1,1))
 // through a DCT-II, both on the coarse octant (an eighth of the fine one).
 // From a fine table already transformed: the eight half-box shifts of it
-// (k_coarse_g0, k_coarse_gt).  Same tables either way.
+// (k_coarse_g0, k_coarse_st).  Same octants either way; they are then tiled
+// for the coarse engine's column layout (all 2 Xc x frequencies, or the half
+// box with the kernels summed over the half-box shift).
 void build_coset_tables(clat_engine* e) {
   clat_engine* s = e->sub;
   s->stream = e->stream;
@@ -1160,27 +1167,38 @@ void build_coset_tables(clat_engine* e) {
     dct2_axis(s, 0, 1.0);
     dct2_axis(s, 1, 1.0);
     dct2_axis(s, 2, sc);
-    tile_conv_table(s);
-    if (s->Gc32)
-      hipLaunchKernelGGL(k_coarse_q<float>, dim3(grid1d(s->q_elems)), dim3(256), 0, e->stream, s->Gt64, s->Xi,
-                         s->Yi, s->Zi, s->gc_c, s->gc_ncb, (float2*)s->Q);
-    else
-      hipLaunchKernelGGL(k_coarse_q<double>, dim3(grid1d(s->q_elems)), dim3(256), 0, e->stream, s->Gt64, s->Xi,
-                         s->Yi, s->Zi, s->gc_c, s->gc_ncb, (double2*)s->Q);
-    LAUNCH_CHECK();
   } else {
     hipLaunchKernelGGL(k_coarse_g0, dim3(grid1d(s->NG())), dim3(256), 0, e->stream, e->G64, e->Xi, e->Yi, e->Zi,
                        s->G64);
     LAUNCH_CHECK();
-    tile_conv_table(s);
-    if (s->Gc32)
-      hipLaunchKernelGGL(k_coarse_gt<float>, dim3(grid1d(s->q_elems)), dim3(256), 0, e->stream, e->G64, e->Xi,
-                         e->Yi, e->Zi, s->gc_c, s->gc_ncb, (float2*)s->Q);
-    else
-      hipLaunchKernelGGL(k_coarse_gt<double>, dim3(grid1d(s->q_elems)), dim3(256), 0, e->stream, e->G64, e->Xi,
-                         e->Yi, e->Zi, s->gc_c, s->gc_ncb, (double2*)s->Q);
+    hipLaunchKernelGGL(k_coarse_st, dim3(grid1d(s->NG())), dim3(256), 0, e->stream, e->G64, e->Xi, e->Yi, e->Zi,
+                       s->Gt64);
     LAUNCH_CHECK();
   }
+  // the conv tables of the current column layout from the two octants
+  const unsigned gq = grid1d(s->q_elems), gg = grid1d(s->gc_elems);
+  if (s->half) {
+    if (s->Gc32) {
+      hipLaunchKernelGGL(k_gconv_tiled_half<float>, dim3(gg), dim3(256), 0, e->stream, s->G64, s->Xi, s->Yi, s->Zi,
+                         s->gc_c, s->gc_ncb, s->Gc32);
+      hipLaunchKernelGGL(k_coarse_q_half<float>, dim3(gq), dim3(256), 0, e->stream, s->Gt64, s->Xi, s->Yi, s->Zi,
+                         s->gc_c, s->gc_ncb, (float2*)s->Q);
+    } else {
+      hipLaunchKernelGGL(k_gconv_tiled_half<double>, dim3(gg), dim3(256), 0, e->stream, s->G64, s->Xi, s->Yi,
+                         s->Zi, s->gc_c, s->gc_ncb, s->Gc64);
+      hipLaunchKernelGGL(k_coarse_q_half<double>, dim3(gq), dim3(256), 0, e->stream, s->Gt64, s->Xi, s->Yi, s->Zi,
+                         s->gc_c, s->gc_ncb, (double2*)s->Q);
+    }
+  } else {
+    tile_conv_table(s);
+    if (s->Gc32)
+      hipLaunchKernelGGL(k_coarse_q<float>, dim3(gq), dim3(256), 0, e->stream, s->Gt64, s->Xi, s->Yi, s->Zi,
+                         s->gc_c, s->gc_ncb, (float2*)s->Q);
+    else
+      hipLaunchKernelGGL(k_coarse_q<double>, dim3(gq), dim3(256), 0, e->stream, s->Gt64, s->Xi, s->Yi, s->Zi,
+                         s->gc_c, s->gc_ncb, (double2*)s->Q);
+  }
+  LAUNCH_CHECK();
   s->initialized = true;
 }
 
@@ -1548,7 +1566,8 @@ std::map<std::vector<int64_t>, int> g_tuned;
 
 std::vector<int64_t> tune_key(const clat_engine* e) {
   return {e->Xi, e->Yi, e->Zi, e->rprec, e->mix, e->sched, e->nb_max, e->n_live, e->n_sites,
-          e->atile ? 1 : 0, e->btile ? 1 : 0, e->n_cu, e->coset ? e->sub->n_live : -1};
+          e->atile ? 1 : 0, e->btile ? 1 : 0, e->n_cu, e->coset ? e->sub->n_live : -1,
+          e->coset && e->sub->half ? 1 : 0};
 }
 
 // The fixed rule's prepares per launch: the engine's own, or on the
@@ -1983,25 +2002,12 @@ bool ensure_sub(clat_engine* e) {
       s->d_bytes = s->nb_max * s->d_set * s->s_d;
       s->D = s->dalloc(s->d_bytes);
     }
-    // cross-coset table and the per-fy factor of its y reflection:
-    // Py[fy] = sign * e^{i pi fy / (2 Yc)}, sign -1 above fy = Yc
-    s->q_elems = (size_t)(s->Yi + 1) * s->gc_ncb * (2 * s->Zi) * s->gc_c;  // rows fz in [0, 2 Zc)
-    s->Q = s->dalloc(s->q_elems * csz);
+    // cross-coset table: a tile per fy in [0, 2 Yc) and column block, rows fz
+    // in [0, 2 Zc), over the 2 Xc columns (the half box carries fewer)
+    s->q_cap = (size_t)2 * s->Yi * 2 * s->Xi * (2 * s->Zi);
+    s->q_elems = (size_t)2 * s->Yi * s->gc_ncb * (2 * s->Zi) * s->gc_c;
+    s->Q = s->dalloc(s->q_cap * csz);
     s->Gt64 = (double*)s->dalloc(s->NG() * sizeof(double));
-    const int ny = 2 * s->Yi;
-    std::vector<double> py((size_t)2 * ny);
-    for (int fy = 0; fy < ny; ++fy) {
-      const double sg = fy > s->Yi ? -1.0 : 1.0, ang = M_PI * (double)fy / (double)ny;
-      py[(size_t)2 * fy] = sg * std::cos(ang);
-      py[(size_t)2 * fy + 1] = sg * std::sin(ang);
-    }
-    s->Py = s->dalloc((size_t)ny * csz);
-    if (csz == sizeof(double2)) {
-      HIPX(hipMemcpy(s->Py, py.data(), py.size() * sizeof(double), hipMemcpyHostToDevice));
-    } else {
-      std::vector<float> pf(py.begin(), py.end());
-      HIPX(hipMemcpy(s->Py, pf.data(), pf.size() * sizeof(float), hipMemcpyHostToDevice));
-    }
   } catch (...) {
     clat_destroy(s);
     e->sub_failed = true;
@@ -2009,6 +2015,22 @@ bool ensure_sub(clat_engine* e) {
   }
   e->sub = s;
   return true;
+}
+
+// The coarse engine's column layout for the current atoms: the half box
+// (x frequencies below Xc only, in whole strips) or all 2 Xc of them.  The
+// buffers and tables were sized for the larger of the two.
+void set_half(clat_engine* s, bool half) {
+  const int C = s->gc_c;
+  s->half = half;
+  s->wcols = half ? (s->Xi + C - 1) / C * C : 2 * s->Xi;
+  s->fmax = half ? s->Xi : 2 * s->Xi;
+  s->gc_w = s->wcols;
+  s->gc_ncb = s->wcols / C;
+  s->gc_elems = (size_t)(s->Yi + 1) * s->gc_ncb * (s->Zi + 1) * C;
+  s->q_elems = (size_t)2 * s->Yi * s->gc_ncb * (2 * s->Zi) * C;
+  s->a_set = s->d_set = (size_t)s->Zi * s->wcols * s->Yi;
+  s->b_set = (size_t)2 * s->Yi * s->wcols * s->Zi;
 }
 
 // Sites of the two-coset path: coset 0 (all coordinates even), then coset 1,
@@ -2315,6 +2337,7 @@ int clat_create(int X, int Y, int Z, int precision, int schedule, clat_engine** 
                  col_fits(e->Yi, 2, (long long)e->Zi * W, e->s_b, e->gc_c, e->s_d);
       if (const char* g = std::getenv("CLAT_ATILE")) e->atile = e->atile && std::atoi(g) != 0;
     }
+    e->wcols = e->fmax = 2 * e->Xi;
     const size_t N = e->N();
     e->f_bytes = N * e->s_in;
     e->v_bytes = N * e->s_v;
@@ -2383,7 +2406,7 @@ int clat_destroy(clat_engine* e) {
                   e->G32, e->Gc64, e->Gc32, e->F, e->V, e->A, e->B, (e->D != e->A) ? e->D : nullptr, e->Fin, e->dpart,
                   e->d_site_atom, e->d_site_cell, e->d_site_ptr, e->d_atom_site, e->d_lines, e->d_live_ptr,
                   e->d_site_ccell, e->d_linemask, e->coeffs, e->rho, e->Vat,
-                  e->part, e->ovl, e->Q, e->Py, e->Gt64};
+                  e->part, e->ovl, e->Q, e->Gt64};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   for (auto ev : e->stage_ev)
@@ -2567,6 +2590,18 @@ int clat_set_atoms(clat_engine* e, const int32_t* cell, int64_t n_atoms, int n_o
     CosetOrder co;
     if (cos) cos = coset_order(e, site_atom, site_cell, co);
     if (cos) cos = ensure_sub(e);
+    // half box: every coarse position on mx + my + mz even (both cosets on
+    // the coarse grid's fcc sublattice, as zincblende's are)
+    bool half = cos;
+    if (const char* g = std::getenv("CLAT_HALFBOX")) half = half && std::atoi(g) != 0;
+    if (half) {
+      const int64_t Xc = e->Xi / 2, Yc = e->Yi / 2;
+      for (int32_t u : co.ucell)
+        if (((u % Xc) + ((u / Xc) % Yc) + (u / (Xc * Yc))) & 1) {
+          half = false;
+          break;
+        }
+    }
     // occupied x-lines: compact list, CSR over it, compact cells, bit mask
     std::vector<int32_t> lines, live_ptr(1, 0), ccell(site_cell.size());
     // y-pass mask: one word per (z, n1) of the y radix split, bit n2 for y = n1 + R1 n2
@@ -2595,6 +2630,7 @@ int clat_set_atoms(clat_engine* e, const int32_t* cell, int64_t n_atoms, int n_o
       // the coarse engine takes the n_u positions as its atoms (one per
       // coarse cell, so its sites are the positions in order); this engine
       // keeps the two-coset site order and no fine line structures
+      set_half(e->sub, half);
       const int rc = clat_set_atoms(e->sub, co.ucell.data(), (int64_t)co.ucell.size(), n_orbitals);
       if (rc != CLAT_OK) throw host::Error(rc, std::string("coarse engine: ") + clat_last_error());
       e->coset = true;

@@ -430,31 +430,98 @@ __global__ void k_coset_samples(const double* __restrict__ G, int Xi, int Yi, in
   }
 }
 
-// The tiled cross-coset table of k_colconv2 from the octant St (Zc+1)(Yc+1)(Xc+1)
-// of DCT-II(Gt) / (8 Xc Yc Zc): for fx, fz in [0, 2Xc), [0, 2Zc) and ry in [0, Yc],
-//   Q = e^{i pi (fx / 2Xc + fz / 2Zc)} s(fx) s(fz) St[r(fz)][ry][r(fx)],
-// r(f) = f up to Xc, else 2Xc - f, with s(f) = -1 past Xc (St is odd about Xc).
+// Half-box form of the two-coset tables, for coarse sites that all lie on
+// mx + my + mz even (the fcc sublattice of the coarse grid, as zincblende's
+// are): a density there has a transform periodic under the half-box shift
+// s = (Xc, Yc, Zc), so only the frequencies kx < Xc are carried, against the
+// kernel summed over the shift, K(k) + K(k + s).  Columns kx in [Xc, W) (W
+// rounded up to the strip width) hold zero.
+__device__ __forceinline__ int refl(int k, int n2) {  // k mod n2, reflected into [0, n2/2]
+  k %= n2;
+  return k <= n2 / 2 ? k : n2 - k;
+}
+
+// within-coset table [ry][column block][rz][c], ry in [0, Yc], rz in [0, Zc]
+// (the summed kernel stays even in y and z), from the octant G0
 template <typename T>
-__global__ void k_coarse_q(const double* __restrict__ St, int Xc, int Yc, int Zc, int C, int ncb,
-                           C2<T>* __restrict__ Q) {
-  const int W = 2 * Xc, Z2 = 2 * Zc;
-  const size_t n = (size_t)(Yc + 1) * ncb * Z2 * C;
+__global__ void k_gconv_tiled_half(const double* __restrict__ G0, int Xc, int Yc, int Zc, int C, int ncb,
+                                   T* __restrict__ out) {
+  const size_t n = (size_t)(Yc + 1) * ncb * (Zc + 1) * C;
+  for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < n; t += (size_t)gridDim.x * blockDim.x) {
+    const int c = (int)(t % C);
+    size_t u = t / C;
+    const int rz = (int)(u % (Zc + 1));
+    u /= (Zc + 1);
+    const int blk = (int)(u % ncb);
+    const int ry = (int)(u / ncb);
+    const int kx = blk * C + c;
+    double v = 0.0;
+    if (kx < Xc) {
+      const size_t X1 = (size_t)Xc + 1, Y1 = (size_t)Yc + 1;
+      v = G0[((size_t)rz * Y1 + ry) * X1 + refl(kx, 2 * Xc)] +
+          G0[((size_t)refl(rz + Zc, 2 * Zc) * Y1 + refl(ry + Yc, 2 * Yc)) * X1 + refl(kx + Xc, 2 * Xc)];
+    }
+    out[t] = (T)v;
+  }
+}
+
+// cross-coset table [fy][column block][fz][c], fy in [0, 2Yc), fz in [0, 2Zc)
+// (k_colconv2), Gt(k) + Gt(k + s) with Gt(k) = e^{i pi (kx/2Xc +
+// ky/2Yc + kz/2Zc)} sign(k) St[r(kz)][r(ky)][r(kx)] on k in the coarse box
+__device__ __forceinline__ double2 coarse_gt_at(const double* __restrict__ St, int Xc, int Yc, int Zc, int kx, int ky,
+                                               int kz) {
+  const int rx = refl(kx, 2 * Xc), ry = refl(ky, 2 * Yc), rz = refl(kz, 2 * Zc);
+  const int neg = (kx > Xc) + (ky > Yc) + (kz > Zc);
+  const double st = ((neg & 1) ? -1.0 : 1.0) * St[((size_t)rz * (Yc + 1) + ry) * (Xc + 1) + rx];
+  double sn, cn;
+  sincospi((double)kx / (2 * Xc) + (double)ky / (2 * Yc) + (double)kz / (2 * Zc), &sn, &cn);
+  return make_double2(st * cn, st * sn);
+}
+
+template <typename T>
+__global__ void k_coarse_q_half(const double* __restrict__ St, int Xc, int Yc, int Zc, int C, int ncb,
+                                C2<T>* __restrict__ Q) {
+  const int Z2 = 2 * Zc, Y2 = 2 * Yc;
+  const size_t n = (size_t)Y2 * ncb * Z2 * C;
   for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < n; t += (size_t)gridDim.x * blockDim.x) {
     const int c = (int)(t % C);
     size_t u = t / C;
     const int fz = (int)(u % Z2);
     u /= Z2;
     const int blk = (int)(u % ncb);
-    const int ry = (int)(u / ncb);
+    const int fy = (int)(u / ncb);
+    const int kx = blk * C + c;
+    C2<T> v = mk<T>((T)0, (T)0);
+    if (kx < Xc) {
+      const double2 a = coarse_gt_at(St, Xc, Yc, Zc, kx, fy, fz);
+      const double2 b = coarse_gt_at(St, Xc, Yc, Zc, kx + Xc, (fy + Yc) % Y2, (fz + Zc) % Z2);
+      v = mk<T>((T)(a.x + b.x), (T)(a.y + b.y));
+    }
+    Q[t] = v;
+  }
+}
+
+// The cross-coset table of k_colconv2 from the octant St (Zc+1)(Yc+1)(Xc+1)
+// of DCT-II(Gt) / (8 Xc Yc Zc), for all 2 Xc x frequencies: Gt(fx, fy, fz) as
+// in coarse_gt_at, tiled [fy][column block][fz][c] (fy in [0, 2Yc), fz in
+// [0, 2Zc), fx = block * C + c < 2 Xc).
+template <typename T>
+__global__ void k_coarse_q(const double* __restrict__ St, int Xc, int Yc, int Zc, int C, int ncb,
+                           C2<T>* __restrict__ Q) {
+  const int W = 2 * Xc, Z2 = 2 * Zc, Y2 = 2 * Yc;
+  const size_t n = (size_t)Y2 * ncb * Z2 * C;
+  for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < n; t += (size_t)gridDim.x * blockDim.x) {
+    const int c = (int)(t % C);
+    size_t u = t / C;
+    const int fz = (int)(u % Z2);
+    u /= Z2;
+    const int blk = (int)(u % ncb);
+    const int fy = (int)(u / ncb);
     const int fx = blk * C + c;
     C2<T> v = mk<T>((T)0, (T)0);
     if (fx < W) {
-      const int rx = fx <= Xc ? fx : W - fx, rz = fz <= Zc ? fz : Z2 - fz;
-      const double sg = ((fx > Xc) != (fz > Zc)) ? -1.0 : 1.0;
-      const double st = sg * St[((size_t)rz * (Yc + 1) + ry) * (Xc + 1) + rx];
-      double sn, cn;
-      sincospi((double)fx / W + (double)fz / Z2, &sn, &cn);
-      v = mk<T>((T)(st * cn), (T)(st * sn));
+      const double2 a = coarse_gt_at(St, Xc, Yc, Zc, fx, fy, fz);
+      v = mk<T>((T)a.x, (T)a.y);
     }
     Q[t] = v;
   }
@@ -554,31 +621,17 @@ __global__ void k_coarse_g0(const double* __restrict__ Gs, int Xi, int Yi, int Z
 
 // p = (1,1,1): the kernel from coset 0 to coset 1,
 //   Gt(q) = e^{i pi (qx/Xi + qy/Yi + qz/Zi)} St(q),  St = the alternating sum,
-// odd about Xc on every axis.  Stored tiled for the coupled conv as
-// [ry][column block][fz][c] (fx = block * C + c < W = Xi, ry in [0, Yc], fz in
-// [0, Zi)), holding e^{i pi (fx/Xi + fz/Zi)} St(fx, ry, fz); the y phase and
-// the sign of the y reflection are applied per block (k_colconv2).
-template <typename T>
-__global__ void k_coarse_gt(const double* __restrict__ Gs, int Xi, int Yi, int Zi, int C, int ncb,
-                            C2<T>* __restrict__ Q) {
-  const int Yc = Yi / 2;
-  const size_t n = (size_t)(Yc + 1) * ncb * Zi * C;
+// St odd about Xc on every axis; its octant (Zc+1)(Yc+1)(Xc+1) (k_coarse_q
+// tiles it for the coupled conv)
+__global__ void k_coarse_st(const double* __restrict__ Gs, int Xi, int Yi, int Zi, double* __restrict__ St) {
+  const int Xc = Xi / 2, Yc = Yi / 2, Zc = Zi / 2;
+  const size_t n = (size_t)(Xc + 1) * (Yc + 1) * (Zc + 1);
   for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < n; t += (size_t)gridDim.x * blockDim.x) {
-    const int c = (int)(t % C);
-    size_t u = t / C;
-    const int fz = (int)(u % Zi);
-    u /= Zi;
-    const int blk = (int)(u % ncb);
-    const int ry = (int)(u / ncb);
-    const int fx = blk * C + c;
-    C2<T> v = mk<T>((T)0, (T)0);
-    if (fx < Xi) {
-      const double st = coarse_sum(Gs, Xi, Yi, Zi, fx, ry, fz, true);
-      double sn, cs;
-      sincospi((double)fx / Xi + (double)fz / Zi, &sn, &cs);
-      v = mk<T>((T)(st * cs), (T)(st * sn));
-    }
-    Q[t] = v;
+    const int qx = (int)(t % (Xc + 1));
+    const size_t u = t / (Xc + 1);
+    const int qy = (int)(u % (Yc + 1));
+    const int qz = (int)(u / (Yc + 1));
+    St[t] = coarse_sum(Gs, Xi, Yi, Zi, qx, qy, qz, true);
   }
 }
 

@@ -38,6 +38,9 @@ struct LineArgs {
   // (byte offsets within one prepare's dense side stay below COL_RANGE)
   int Yl, lgsw;
   int lz, ly, strip;
+  // frequencies carried on the dense side: f < fmax (the rest are not
+  // stored by the forward pass and read as zero by the inverse one)
+  int fmax;
   long long in_bs, out_bs;          // element stride between batched prepares on the dense side
 };
 
@@ -189,9 +192,12 @@ __global__ __launch_bounds__((line_threads<C, R1, R2, NSUB>())) void k_linepass(
       const int lb = line_base(a, a.lines[b0 + c2]);
       int off[R1];
       freq_offs<R1, R2 * NSUB>(a, k2 * NSUB + s2, off);
+      const int f0 = k2 * NSUB + s2;
 #pragma unroll
-      for (int k1 = 0; k1 < R1; ++k1)
-        stg_buf<ST_XFWD>(ob, (uint32_t)(lb + off[k1]) * (uint32_t)sizeof(C2<TO>), cvt<TC, TO>(w[k1]));
+      for (int k1 = 0; k1 < R1; ++k1) {
+        const uint32_t o = f0 + k1 * R2 * NSUB < a.fmax ? (uint32_t)(lb + off[k1]) * (uint32_t)sizeof(C2<TO>) : COL_OOB;
+        stg_buf<ST_XFWD>(ob, o, cvt<TC, TO>(w[k1]));
+      }
     }
   } else {
     // ---------------------------------------------------------- P2 <- D
@@ -203,9 +209,12 @@ __global__ __launch_bounds__((line_threads<C, R1, R2, NSUB>())) void k_linepass(
       const int lb = line_base(a, a.lines[b0 + c2]);
       int off[R1];
       freq_offs<R1, R2 * NSUB>(a, k2 * NSUB + s2, off);
+      const int f0 = k2 * NSUB + s2;
 #pragma unroll
-      for (int k1 = 0; k1 < R1; ++k1)
-        w[k1] = cvt<TI, TC>(ldg_stream_off(ib, (uint32_t)(lb + off[k1]) * (uint32_t)sizeof(C2<TI>)));
+      for (int k1 = 0; k1 < R1; ++k1) {
+        const uint32_t o = f0 + k1 * R2 * NSUB < a.fmax ? (uint32_t)(lb + off[k1]) * (uint32_t)sizeof(C2<TI>) : COL_OOB;
+        w[k1] = cvt<TI, TC>(ldg_stream_off(ib, o));
+      }
       rdft<R1, true, TC>(w);
       const uint32_t tstep = (uint32_t)(2 * k2 + s2) * SC;
       // twiddles W_2L^{m q}, q = 2 k2 + s2.  FP64: the powers of two m come
