@@ -2849,6 +2849,7 @@ int clat_engine_kernels(clat_engine* e, uint32_t* flags) {
   if (e->btile) f |= CLAT_KERNELS_B_STRIPS;
   if (e->atile) f |= CLAT_KERNELS_AD_STRIPS;
   if (e->coset) f |= CLAT_KERNELS_COSETS;
+  if (e->coset && e->sub->half) f |= CLAT_KERNELS_HALF_BOX;
   *flags = f;
   return CLAT_OK;
 }

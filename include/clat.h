@@ -265,8 +265,12 @@ int clat_engine_info(clat_engine* e, int dim[3], int64_t* device_bytes);
  * as the zincblende and diamond lattices on their a/4 grid do: the
  * convolution then runs on the two cosets of the coarse grid with half the
  * points per axis, coupled by a 2 x 2 kernel; CLAT_COSETS=0 in the
- * environment keeps the single-grid path).  An introspection hook with no
- * counterpart in the reference: every combination gives the same results. */
+ * environment keeps the single-grid path), and its half-box form (every
+ * coarse position on mx + my + mz even, as zincblende's are: only the x
+ * frequencies below half the coarse box are carried, against kernels summed
+ * over the half-box shift; CLAT_HALFBOX=0 keeps all of them).  An
+ * introspection hook with no counterpart in the reference: every combination
+ * gives the same results. */
 #define CLAT_KERNELS_TWO_LEVEL 1u
 #define CLAT_KERNELS_COLUMN 2u
 #define CLAT_KERNELS_LINE 4u
@@ -274,6 +278,7 @@ int clat_engine_info(clat_engine* e, int dim[3], int64_t* device_bytes);
 #define CLAT_KERNELS_B_STRIPS 16u
 #define CLAT_KERNELS_AD_STRIPS 32u
 #define CLAT_KERNELS_COSETS 64u
+#define CLAT_KERNELS_HALF_BOX 128u
 int clat_engine_kernels(clat_engine* e, uint32_t* flags);
 
 /* ---- planning ---------------------------------------------------------- */

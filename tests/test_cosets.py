@@ -126,3 +126,53 @@ def test_two_coset_convolution_matches_oracle(dims):
     # the cross terms carry real weight here: without them coset 1 is far off
     v1_alone = inv(g0 * r1)
     assert np.max(np.abs(v1_alone - Vo[1::2, 1::2, 1::2])) > 1e-3 * scale
+
+
+@pytest.mark.parametrize("dims", [(16, 12, 20), (8, 8, 12)])
+def test_half_box_matches_oracle(dims):
+    """Sites on the coarse grid's fcc sublattice (mx + my + mz even, in both
+    cosets): the density's coarse transform is periodic under the half-box
+    shift s = (Xc, Yc, Zc), so carrying only kx < Xc against the kernels summed
+    over the shift, K(k) + K(k + s) (kernels.hpp: k_gconv_tiled_half,
+    k_coarse_q_half), gives the potential at every site.  One site with an
+    odd coarse sum breaks it (why the engine checks every position)."""
+    X, Y, Z = dims
+    Xc, Yc, Zc = X // 2, Y // 2, Z // 2
+    rng = np.random.default_rng(X + 7 * Y + 13 * Z)
+    zz, yy, xx = np.meshgrid(np.arange(Z), np.arange(Y), np.arange(X), indexing="ij")
+    p = xx % 2
+    fcc = ((xx - p) // 2 + (yy - p) // 2 + (zz - p) // 2) % 2 == 0
+    on = ((xx % 2) == (yy % 2)) & ((yy % 2) == (zz % 2)) & fcc & (rng.random((Z, Y, X)) < 0.8)
+    Gs = O.ghat(O.gtable(X, Y, Z, (0.7, 0.9, 1.1), 0.3, 2.0))
+    g0, gt = coarse_tables(Gs)
+
+    def per(T):
+        return T + np.roll(T, (-Zc, -Yc, -Xc), axis=(0, 1, 2))
+
+    def half_box(F):
+        def fwd(r):
+            P = np.zeros((Z, Y, X), complex)
+            P[:Zc, :Yc, :Xc] = r
+            return np.fft.fftn(P)
+
+        def inv(P):
+            Q = np.zeros((Z, Y, X), complex)
+            Q[:, :, :Xc] = P[:, :, :Xc]
+            return np.fft.ifftn(Q)[:Zc, :Yc, :Xc] * (X * Y * Z)
+
+        r0, r1 = fwd(F[0::2, 0::2, 0::2]), fwd(F[1::2, 1::2, 1::2])
+        return inv(per(g0) * r0 + per(np.conj(gt)) * r1), inv(per(gt) * r0 + per(g0) * r1)
+
+    F = np.zeros((Z, Y, X), complex)
+    F[on] = rng.standard_normal(on.sum()) + 1j * rng.standard_normal(on.sum())
+    Vo = O.prepare(F, Gs)
+    v0, v1 = half_box(F)
+    m0, m1 = on[0::2, 0::2, 0::2], on[1::2, 1::2, 1::2]
+    scale = np.max(np.abs(Vo))
+    assert np.max(np.abs(v0 - Vo[0::2, 0::2, 0::2])[m0]) < 1e-12 * scale
+    assert np.max(np.abs(v1 - Vo[1::2, 1::2, 1::2])[m1]) < 1e-12 * scale
+    # one extra site at an odd coarse sum (coarse (1, 0, 0) of coset 0)
+    F[0, 0, 2] = 1.0
+    Vo = O.prepare(F, Gs)
+    v0, _ = half_box(F)
+    assert np.max(np.abs(v0 - Vo[0::2, 0::2, 0::2])[m0]) > 1e-6 * scale

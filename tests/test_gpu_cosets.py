@@ -50,7 +50,7 @@ def test_cosets_match_oracle_and_single_grid(prec, box, monkeypatch):
     print("case cosets", box, prec, w["dims"], flush=True)
     a = _engine(w, w["cells"], prec, 4, w["states"], True, monkeypatch)
     b = _engine(w, w["cells"], prec, 4, w["states"], False, monkeypatch)
-    assert "cosets" in a.kernels() and "cosets" not in b.kernels()
+    assert {"cosets", "half_box"} <= a.kernels() and "cosets" not in b.kernels()
     ga, gb = a.compute_integrals(quads), b.compute_integrals(quads)
     pa, pb = a.potential_at_atoms(2), b.potential_at_atoms(2)
     a.close()
@@ -184,3 +184,47 @@ def test_coset_table_routes_agree(prec, monkeypatch):
     else:
         assert np.max(rel_err(sampled, shifted)) < 1e-5
         check_fp32x_model(sampled, want, quads, "sampled tables fp32x")
+
+
+@pytest.mark.parametrize("prec", [clat.PREC_FP64, clat.PREC_FP32X], ids=["fp64", "fp32x"])
+def test_cosets_full_box(prec, monkeypatch):
+    """Both cosets, all 2 Xc x frequencies: an extra atom at fine (2, 0, 0)
+    (coset 0, coarse sum odd) keeps the engine off the half box, and
+    CLAT_HALFBOX=0 does the same for plain zincblende.  Integrals against the
+    oracle and, for zincblende, against the half-box form."""
+    nh, ne = 2, 1
+    w = zb.workload(16, 16, 16, nh, ne, config=7, sigma_frac=0.2, n_orb=4)
+    X, Y, Z = w["dims"]
+    quads = clat.pattern_enumerate("****", nh + ne, nh)
+    idx = np.concatenate([w["idx"], [[2, 0, 0]]])
+    assert int(zb.cells_of(idx[-1:], (X, Y, Z))[0]) not in set(w["cells"].tolist())
+    cells = zb.cells_of(idx, (X, Y, Z))
+    rng = np.random.default_rng(11)
+    states = [np.concatenate([s, (rng.standard_normal((1, 4)) + 1j * rng.standard_normal((1, 4))) * 0.05])
+              for s in w["states"]]
+    eng = _engine(w, cells, prec, 4, states, True, monkeypatch)
+    assert "cosets" in eng.kernels() and "half_box" not in eng.kernels()
+    got = eng.compute_integrals(quads)
+    eng.close()
+    want = _oracle(w, cells, states, quads, nh + ne)
+    if prec == clat.PREC_FP64:
+        check(got, want, prec, quads)
+    else:
+        check_fp32x_model(got, want, quads, "full box fp32x")
+    monkeypatch.setenv("CLAT_HALFBOX", "0")
+    full = _engine(w, w["cells"], prec, 4, w["states"], True, None)
+    assert "cosets" in full.kernels() and "half_box" not in full.kernels()
+    gf = full.compute_integrals(quads)
+    full.close()
+    monkeypatch.delenv("CLAT_HALFBOX")
+    half = _engine(w, w["cells"], prec, 4, w["states"], True, None)
+    assert "half_box" in half.kernels()
+    gh = half.compute_integrals(quads)
+    half.close()
+    want = _oracle(w, w["cells"], w["states"], quads, nh + ne)
+    if prec == clat.PREC_FP64:
+        check(gf, want, prec, quads)
+        assert np.max(rel_err(gh, gf)) < 1e-12
+    else:
+        check_fp32x_model(gf, want, quads, "full box fp32x")
+        check_fp32x_model(gh, want, quads, "half box fp32x")
