@@ -518,17 +518,14 @@ __global__ __launch_bounds__((col_threads<C, R1>())) void k_colconv2(const C2<TI
     __syncthreads();
     // -------------------------------------------------------------- P2
     if (r < R2) {
-      CT v[R1], w[R1];
-#pragma unroll
-      for (int n1 = 0; n1 < R1; ++n1) {
-        v[n1] = lds0[(r * R1 + n1) * C + c];
-        w[n1] = lds1[(r * R1 + n1) * C + c];
-      }
-      rdft<R1, false, TC>(v);  // v[k1] = R0_s[R2 k1 + k2]
-      rdft<R1, false, TC>(w);
+      // the thread's table values first: their loads are in flight during
+      // the LDS reads and the forward transforms (issued one pair at a time
+      // next to their use, each waited for in turn)
       // fz = 2 (R2 k1 + k2) + s; g0 row rz = min(fz, 2L - fz), Q row fz
       const int q = 2 * r + s;
       const uint32_t lo = (uint32_t)q * (C * sizeof(TC));
+      TC gv[R1];
+      CT qv[R1];
 #pragma unroll
       for (int k1 = 0; k1 < R1; ++k1) {
         constexpr uint32_t RB = C * sizeof(TC);
@@ -539,8 +536,21 @@ __global__ __launch_bounds__((col_threads<C, R1>())) void k_colconv2(const C2<TI
           const int fz = 2 * R2 * k1 + q;
           off = (uint32_t)min(fz, 2 * L - fz) * RB;
         }
-        const TC g = ldg_off(Gb, off + gcol);
-        const CT gt = ldg_off(Qb, (uint32_t)(2 * R2 * k1 + q) * (C * SC) + qcol);
+        gv[k1] = ldg_off(Gb, off + gcol);
+        qv[k1] = ldg_off(Qb, (uint32_t)(2 * R2 * k1 + q) * (C * SC) + qcol);
+      }
+      CT v[R1], w[R1];
+#pragma unroll
+      for (int n1 = 0; n1 < R1; ++n1) {
+        v[n1] = lds0[(r * R1 + n1) * C + c];
+        w[n1] = lds1[(r * R1 + n1) * C + c];
+      }
+      rdft<R1, false, TC>(v);  // v[k1] = R0_s[R2 k1 + k2]
+      rdft<R1, false, TC>(w);
+#pragma unroll
+      for (int k1 = 0; k1 < R1; ++k1) {
+        const TC g = gv[k1];
+        const CT gt = qv[k1];
         const CT a0 = v[k1], a1 = w[k1];
         v[k1] = mk<TC>(g * a0.x + gt.x * a1.x + gt.y * a1.y, g * a0.y + gt.x * a1.y - gt.y * a1.x);
         w[k1] = mk<TC>(gt.x * a0.x - gt.y * a0.y + g * a1.x, gt.x * a0.y + gt.y * a0.x + g * a1.y);
