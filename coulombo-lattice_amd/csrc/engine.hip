@@ -323,6 +323,7 @@ struct clat_engine {
   void* Q = nullptr;               // cross-coset kernel table (k_coarse_q / k_coarse_q_half)
   size_t q_cap = 0;                // its capacity in elements
   double* Gt64 = nullptr;          // octant of the cross-coset kernel: samples, then DCT-II
+  double2* ph64 = nullptr;         // e^{i pi k / (2 L)}, k < 2 L, per axis (the cross-coset phases)
   // half-box form (every coarse site on mx + my + mz even): x frequencies
   // f < fmax carried in wcols columns (the strip width's multiple), kernels
   // summed over the half-box shift; otherwise wcols = fmax = 2 Xi
@@ -1181,22 +1182,22 @@ void build_coset_tables(clat_engine* e) {
     if (s->Gc32) {
       hipLaunchKernelGGL(k_gconv_tiled_half<float>, dim3(gg), dim3(256), 0, e->stream, s->G64, s->Xi, s->Yi, s->Zi,
                          s->gc_c, s->gc_ncb, s->Gc32);
-      hipLaunchKernelGGL(k_coarse_q_half<float>, dim3(gq), dim3(256), 0, e->stream, s->Gt64, s->Xi, s->Yi, s->Zi,
-                         s->gc_c, s->gc_ncb, (float2*)s->Q);
+      hipLaunchKernelGGL(k_coarse_q_half<float>, dim3(gq), dim3(256), 0, e->stream, s->Gt64, s->ph64, s->Xi, s->Yi,
+                         s->Zi, s->gc_c, s->gc_ncb, (float2*)s->Q);
     } else {
       hipLaunchKernelGGL(k_gconv_tiled_half<double>, dim3(gg), dim3(256), 0, e->stream, s->G64, s->Xi, s->Yi,
                          s->Zi, s->gc_c, s->gc_ncb, s->Gc64);
-      hipLaunchKernelGGL(k_coarse_q_half<double>, dim3(gq), dim3(256), 0, e->stream, s->Gt64, s->Xi, s->Yi, s->Zi,
-                         s->gc_c, s->gc_ncb, (double2*)s->Q);
+      hipLaunchKernelGGL(k_coarse_q_half<double>, dim3(gq), dim3(256), 0, e->stream, s->Gt64, s->ph64, s->Xi, s->Yi,
+                         s->Zi, s->gc_c, s->gc_ncb, (double2*)s->Q);
     }
   } else {
     tile_conv_table(s);
     if (s->Gc32)
-      hipLaunchKernelGGL(k_coarse_q<float>, dim3(gq), dim3(256), 0, e->stream, s->Gt64, s->Xi, s->Yi, s->Zi,
+      hipLaunchKernelGGL(k_coarse_q<float>, dim3(gq), dim3(256), 0, e->stream, s->Gt64, s->ph64, s->Xi, s->Yi, s->Zi,
                          s->gc_c, s->gc_ncb, (float2*)s->Q);
     else
-      hipLaunchKernelGGL(k_coarse_q<double>, dim3(gq), dim3(256), 0, e->stream, s->Gt64, s->Xi, s->Yi, s->Zi,
-                         s->gc_c, s->gc_ncb, (double2*)s->Q);
+      hipLaunchKernelGGL(k_coarse_q<double>, dim3(gq), dim3(256), 0, e->stream, s->Gt64, s->ph64, s->Xi, s->Yi,
+                         s->Zi, s->gc_c, s->gc_ncb, (double2*)s->Q);
   }
   LAUNCH_CHECK();
   s->initialized = true;
@@ -2008,6 +2009,18 @@ bool ensure_sub(clat_engine* e) {
     s->q_elems = (size_t)2 * s->Yi * s->gc_ncb * (2 * s->Zi) * s->gc_c;
     s->Q = s->dalloc(s->q_cap * csz);
     s->Gt64 = (double*)s->dalloc(s->NG() * sizeof(double));
+    {
+      const int dims[3] = {s->Xi, s->Yi, s->Zi};
+      std::vector<double> ph;
+      for (int d = 0; d < 3; ++d)
+        for (int k = 0; k < 2 * dims[d]; ++k) {
+          const double a = M_PI * (double)k / (2.0 * dims[d]);
+          ph.push_back(std::cos(a));
+          ph.push_back(std::sin(a));
+        }
+      s->ph64 = (double2*)s->dalloc(ph.size() * sizeof(double));
+      HIPX(hipMemcpy(s->ph64, ph.data(), ph.size() * sizeof(double), hipMemcpyHostToDevice));
+    }
   } catch (...) {
     clat_destroy(s);
     e->sub_failed = true;
@@ -2406,7 +2419,7 @@ int clat_destroy(clat_engine* e) {
                   e->G32, e->Gc64, e->Gc32, e->F, e->V, e->A, e->B, (e->D != e->A) ? e->D : nullptr, e->Fin, e->dpart,
                   e->d_site_atom, e->d_site_cell, e->d_site_ptr, e->d_atom_site, e->d_lines, e->d_live_ptr,
                   e->d_site_ccell, e->d_linemask, e->coeffs, e->rho, e->Vat,
-                  e->part, e->ovl, e->Q, e->Gt64};
+                  e->part, e->ovl, e->Q, e->Gt64, e->ph64};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   for (auto ev : e->stage_ev)

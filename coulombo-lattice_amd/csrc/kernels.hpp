@@ -468,19 +468,20 @@ __global__ void k_gconv_tiled_half(const double* __restrict__ G0, int Xc, int Yc
 // cross-coset table [fy][column block][fz][c], fy in [0, 2Yc), fz in [0, 2Zc)
 // (k_colconv2), Gt(k) + Gt(k + s) with Gt(k) = e^{i pi (kx/2Xc +
 // ky/2Yc + kz/2Zc)} sign(k) St[r(kz)][r(ky)][r(kx)] on k in the coarse box
-__device__ __forceinline__ double2 coarse_gt_at(const double* __restrict__ St, int Xc, int Yc, int Zc, int kx, int ky,
-                                               int kz) {
+// ph: the per-axis phases e^{i pi k / (2 Xc)}, k < 2 Xc, then the same for y
+// and z (2 Xc + 2 Yc + 2 Zc entries)
+__device__ __forceinline__ double2 coarse_gt_at(const double* __restrict__ St, const double2* __restrict__ ph, int Xc,
+                                               int Yc, int Zc, int kx, int ky, int kz) {
   const int rx = refl(kx, 2 * Xc), ry = refl(ky, 2 * Yc), rz = refl(kz, 2 * Zc);
   const int neg = (kx > Xc) + (ky > Yc) + (kz > Zc);
   const double st = ((neg & 1) ? -1.0 : 1.0) * St[((size_t)rz * (Yc + 1) + ry) * (Xc + 1) + rx];
-  double sn, cn;
-  sincospi((double)kx / (2 * Xc) + (double)ky / (2 * Yc) + (double)kz / (2 * Zc), &sn, &cn);
-  return make_double2(st * cn, st * sn);
+  const double2 p = cmul<double>(cmul<double>(ph[kx], ph[2 * Xc + ky]), ph[2 * Xc + 2 * Yc + kz]);
+  return make_double2(st * p.x, st * p.y);
 }
 
 template <typename T>
-__global__ void k_coarse_q_half(const double* __restrict__ St, int Xc, int Yc, int Zc, int C, int ncb,
-                                C2<T>* __restrict__ Q) {
+__global__ void k_coarse_q_half(const double* __restrict__ St, const double2* __restrict__ ph, int Xc, int Yc, int Zc,
+                                int C, int ncb, C2<T>* __restrict__ Q) {
   const int Z2 = 2 * Zc, Y2 = 2 * Yc;
   const size_t n = (size_t)Y2 * ncb * Z2 * C;
   for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < n; t += (size_t)gridDim.x * blockDim.x) {
@@ -493,8 +494,8 @@ __global__ void k_coarse_q_half(const double* __restrict__ St, int Xc, int Yc, i
     const int kx = blk * C + c;
     C2<T> v = mk<T>((T)0, (T)0);
     if (kx < Xc) {
-      const double2 a = coarse_gt_at(St, Xc, Yc, Zc, kx, fy, fz);
-      const double2 b = coarse_gt_at(St, Xc, Yc, Zc, kx + Xc, (fy + Yc) % Y2, (fz + Zc) % Z2);
+      const double2 a = coarse_gt_at(St, ph, Xc, Yc, Zc, kx, fy, fz);
+      const double2 b = coarse_gt_at(St, ph, Xc, Yc, Zc, kx + Xc, (fy + Yc) % Y2, (fz + Zc) % Z2);
       v = mk<T>((T)(a.x + b.x), (T)(a.y + b.y));
     }
     Q[t] = v;
@@ -506,8 +507,8 @@ __global__ void k_coarse_q_half(const double* __restrict__ St, int Xc, int Yc, i
 // in coarse_gt_at, tiled [fy][column block][fz][c] (fy in [0, 2Yc), fz in
 // [0, 2Zc), fx = block * C + c < 2 Xc).
 template <typename T>
-__global__ void k_coarse_q(const double* __restrict__ St, int Xc, int Yc, int Zc, int C, int ncb,
-                           C2<T>* __restrict__ Q) {
+__global__ void k_coarse_q(const double* __restrict__ St, const double2* __restrict__ ph, int Xc, int Yc, int Zc, int C,
+                           int ncb, C2<T>* __restrict__ Q) {
   const int W = 2 * Xc, Z2 = 2 * Zc, Y2 = 2 * Yc;
   const size_t n = (size_t)Y2 * ncb * Z2 * C;
   for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < n; t += (size_t)gridDim.x * blockDim.x) {
@@ -520,7 +521,7 @@ __global__ void k_coarse_q(const double* __restrict__ St, int Xc, int Yc, int Zc
     const int fx = blk * C + c;
     C2<T> v = mk<T>((T)0, (T)0);
     if (fx < W) {
-      const double2 a = coarse_gt_at(St, Xc, Yc, Zc, fx, fy, fz);
+      const double2 a = coarse_gt_at(St, ph, Xc, Yc, Zc, fx, fy, fz);
       v = mk<T>((T)a.x, (T)a.y);
     }
     Q[t] = v;
