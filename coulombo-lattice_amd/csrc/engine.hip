@@ -1637,6 +1637,11 @@ int choose_batch(clat_engine* e, const Units& units, const std::vector<int>& slo
     e->tune_ms[i] = best[i];
   }
   for (size_t i = cand.size(); i < 4; ++i) e->tune_ms[i] = 0.0;
+  // the rule's count stays unless another is more than 2 % faster (the trials
+  // of neighbouring counts often differ by less than their run-to-run noise)
+  const int rule = batch_rule(e);
+  for (size_t i = 0; i < cand.size(); ++i)
+    if (cand[i] == rule && best[i] <= 1.02 * best[pick]) pick = i;
   e->tuned_nb = cand[pick];
   std::lock_guard<std::mutex> lock(g_tune_mu);
   g_tuned[key] = e->tuned_nb;

@@ -138,7 +138,7 @@ def main():
         fetch, write = counters(a.fetch, "FETCH_SIZE"), counters(a.write, "WRITE_SIZE")
         per_kind = {}
         for k in fetch:
-            kd = kind_of(k) if ("k_pass2<" in k or "k_colpass<" in k or "k_linepass<" in k) \
+            kd = kind_of(k) if ("k_pass2<" in k or "k_colpass<" in k or "k_linepass<" in k or "k_colconv2<" in k) \
                 else short(k).replace("clat::", "")
             fk, wk = fetch[k], write.get(k, 0.0)
             per_kind[kd] = {"kernel": short(k), "fetch_kib": fk, "write_kib": wk,

@@ -95,7 +95,6 @@ def test_measured_batch_is_cached_and_bit_identical(monkeypatch):
     va = a.compute_integrals(quads)
     ta = a.tuning()
     assert ta["batch"] in (1, 2, 4, 8) and len(ta["trial_ms"]) >= 2, ta
-    assert ta["batch"] == min(ta["trial_ms"], key=ta["trial_ms"].get)
     b = _engine(w, clat.PREC_FP32X)
     vb = b.compute_integrals(quads)
     tb = b.tuning()
@@ -103,6 +102,10 @@ def test_measured_batch_is_cached_and_bit_identical(monkeypatch):
     c = _engine(w, clat.PREC_FP32X)
     c.set_tuning(False)
     vc = c.compute_integrals(quads)
+    # the fastest trial, or the fixed rule's count when within 2 % of it
+    trials, rule = ta["trial_ms"], c.tuning()["batch"]
+    fastest = min(trials, key=trials.get)
+    assert ta["batch"] == fastest or (ta["batch"] == rule and trials[rule] <= 1.02 * trials[fastest]), (ta, rule)
     monkeypatch.setenv("CLAT_BATCH", "1")
     d = _engine(w, clat.PREC_FP32X)
     vd = d.compute_integrals(quads)
