@@ -339,6 +339,8 @@ struct clat_engine {
   int ovl_mode = 0;  // overlap kernel: 0 by size, 1 vector FMA, 2 FP64 MFMA, 3 vector FMA in 48 x 48 tiles,
                     // 4 the same with every site sum Kahan-compensated
   bool table_pending = false;           // stage_ms[0] of the last table build not read yet
+  bool table_deferred = false;          // a set interaction whose table is not enqueued yet
+  GArgs gdefer{};                       // its arguments
   size_t a_set = 0, b_set = 0, d_set = 0;  // elements of A, B, D per prepare
   bool xline = false;                // x passes on the line kernel, y passes skip empty lines
   int n_states = 0, cap_states = 0;
@@ -1037,7 +1039,10 @@ void fp32x_prepare(clat_engine* e, const double2* si, double2* so) {
 void ensure_fine_tables(clat_engine* e);
 
 // dense F -> dense V
+bool ensure_table(clat_engine* e);
+
 void prepare_dense(clat_engine* e) {
+  ensure_table(e);
   ensure_dense(e);
   ensure_fine_tables(e);
   if (e->fast) {
@@ -1256,15 +1261,30 @@ GArgs base_gargs(const clat_engine* e) {
   return g;
 }
 
-// G table + DCT-I on the engine stream, timed into stage_ms[0]
+// A new interaction: its G table + DCT-I are enqueued by the first call
+// that needs them (ensure_table), which in an integral run is after the
+// run's pair densities, so the device has work while the host launches the
+// table's kernels
 void build_table(clat_engine* e, GArgs g) {
+  e->gdefer = g;
+  e->table_deferred = true;
+  e->initialized = true;
+  e->prepared = false;
+}
+
+// the deferred G table + DCT-I on the engine stream, timed into stage_ms[0];
+// true when it was enqueued by this call
+bool ensure_table(clat_engine* e) {
+  if (!e->table_deferred) return false;
   HIPX(hipEventRecord(e->stage_ev[0], e->stream));
-  hipLaunchKernelGGL(k_gtable, dim3(grid1d(e->NG())), dim3(256), 0, e->stream, e->G64, g);
+  hipLaunchKernelGGL(k_gtable, dim3(grid1d(e->NG())), dim3(256), 0, e->stream, e->G64, e->gdefer);
   LAUNCH_CHECK();
   finish_interaction(e);
+  e->table_deferred = false;
   HIPX(hipEventRecord(e->stage_ev[5], e->stream));
   // no host wait here: the next call that waits for the stream reads the time
   e->table_pending = true;
+  return true;
 }
 
 // time of the last table build, once the stream has passed it
@@ -1405,6 +1425,7 @@ void potential_run(clat_engine* e, int f, double2* rho, double2* vs, double2* va
     hipLaunchKernelGGL(k_self_density, dim3(grid1d(e->n_sites)), dim3(256), 0, e->stream, e->coeffs, e->n_sites,
                        e->n_orb, f, rho);
     LAUNCH_CHECK();
+    ensure_table(e);
     prepare_sites(e, rho, vs);
   }
   if (e->n_atoms > 0) {
@@ -1747,7 +1768,7 @@ void device_stage(clat_engine* e, const Units& units, const std::vector<int>& sl
 // ===========================================================================
 struct RunPlan {
   int P = 0, nL = 0, nbt = 1;
-  bool stale = false, device = false;
+  bool stale = false, device = false, table = false;
   std::vector<int> slot_of, slots;
   Units units;
   OvlCfg oc;
@@ -1760,6 +1781,7 @@ void run_begin(clat_engine* e, const int32_t* quads, int64_t n, int flags, RunPl
   e->rho_states = -1;  // the pair densities are part of every run's compute stage
   build_products(e);
   HIPX(hipEventRecord(e->stage_ev[2], e->stream));
+  rp.table = ensure_table(e);  // a new interaction's table, queued behind the densities
   // the stale-values bookkeeping walks the steps in execution order; without
   // it every value depends on its own step only and id order serves
   rp.stale = (flags & CLAT_FLAG_STALE_VALUES) != 0;
@@ -1872,7 +1894,7 @@ void run_finish(clat_engine* e, const int32_t* quads, int64_t n, const RunPlan& 
   }
   float ms;
   HIPX(hipEventElapsedTime(&ms, e->stage_ev[1], e->stage_ev[2])); e->stage_ms[1] = ms;
-  HIPX(hipEventElapsedTime(&ms, e->stage_ev[2], e->stage_ev[3])); e->stage_ms[2] = ms;
+  HIPX(hipEventElapsedTime(&ms, e->stage_ev[rp.table ? 5 : 2], e->stage_ev[3])); e->stage_ms[2] = ms;
   HIPX(hipEventElapsedTime(&ms, e->stage_ev[3], e->stage_ev[4])); e->stage_ms[3] = ms;
   HIPX(hipEventElapsedTime(&ms, e->stage_ev[1], e->stage_ev[4])); e->stage_ms[4] = ms;
 }
@@ -2484,6 +2506,7 @@ int clat_set_interaction_thomas_fermi(clat_engine* e, const double step[3], doub
 int clat_set_interaction_table(clat_engine* e, const double* G) {
   if (!e || !G) return set_err(CLAT_EINVAL, "invalid arguments");
   return guarded([&] {
+    e->table_deferred = false;  // this table replaces a set interaction not built yet
     const size_t nu = (size_t)(e->X + 1) * (e->Y + 1) * (e->Z + 1);
     double* tmp = (double*)e->dalloc(nu * sizeof(double));
     hipError_t ce = hipMemcpyAsync(tmp, G, nu * sizeof(double), hipMemcpyHostToDevice, e->stream);
@@ -2661,8 +2684,8 @@ int clat_set_atoms(clat_engine* e, const int32_t* cell, int64_t n_atoms, int n_o
       live_ptr.assign(1, 0);
       ccell.assign(site_atom.size(), 0);
       std::fill(mask.begin(), mask.end(), 0u);
-      if (e->initialized) build_coset_tables(e);
-    } else if (e->initialized) {
+      if (e->initialized && !e->table_deferred) build_coset_tables(e);
+    } else if (e->initialized && !e->table_deferred) {
       ensure_fine_tables(e);
     }
     e->n_sites = (int64_t)site_atom.size();

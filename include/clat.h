@@ -135,7 +135,10 @@ int clat_destroy(clat_engine* e);
  * (Interaction.hpp:32-51, Interaction.cpp:8-14, CoulombCalculator.cpp:57-103):
  * builds G(ix,iy,iz) = 14.39963737103201/(dielectric*r) on the
  * (X+1)(Y+1)(Z+1) first octant with G(0,0,0) = onsite, then its 3-D DCT-I
- * divided by 8*X*Y*Z -- all in FP64 on the GPU. */
+ * divided by 8*X*Y*Z -- all in FP64 on the GPU.  The table is built by the
+ * first call that needs it (clat_prepare, clat_compute_integrals, after that
+ * call's pair densities, clat_compute_potential_at_atoms*), so a failure of
+ * its kernels is reported there; the same holds for the Thomas-Fermi form. */
 int clat_set_interaction_simple(clat_engine* e, const double step[3],
                                 double onsite, double dielectric);
 
@@ -230,7 +233,8 @@ int clat_compute_potential_at_atoms_noalloc(clat_engine* e, int state, double* v
 /* Stage times in milliseconds from HIP events on the engine stream:
  * [0] the last interaction build (G table + DCT-I + scaling), and for the
  * last clat_compute_integrals call [1] pair densities, [2] all prepares,
- * [3] overlaps, [4] the whole call. */
+ * [3] overlaps, [4] the whole call (with [0] when that call built the
+ * table). */
 int clat_last_stage_ms(clat_engine* e, double ms[5]);
 
 /* When enabled, every FFT pass launch is bracketed by HIP events on the
