@@ -28,7 +28,7 @@ SCHED_SHARED, SCHED_ROUNDS = 0, 1
 FLAG_STALE_VALUES = 1
 # bits of clat_engine_kernels
 KERNEL_FLAGS = {"two_level": 1, "column": 2, "line": 4, "conv_tiled": 8, "b_strips": 16, "ad_strips": 32,
-                "cosets": 64, "half_box": 128}
+                "cosets": 64, "half_box": 128, "half_lines": 256}
 
 _dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
 _ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")

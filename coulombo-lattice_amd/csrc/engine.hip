@@ -329,6 +329,15 @@ struct clat_engine {
   // summed over the half-box shift; otherwise wcols = fmax = 2 Xi
   bool half = false;
   int wcols = 0, fmax = 0;
+  // half-length x lines (coarse engine of a half box): every site has
+  // x = (y + z) mod 2, so a line is carried as its Xi / 2 sites (radix pair
+  // hx of Xi / 2) and the line kernels add the x parity's phase.  The x
+  // inverse (FP64 in FP32X) always takes them; the FP32X forward keeps full
+  // lines, whose complex64 rounding pattern keeps the 1M-atom workload's
+  // smallest integrals within 1e-5 of FP64 (9.1e-6 against 1.03e-5 on half
+  // lines: the noise of FP32 arithmetic at 1e-8 of the largest integral)
+  bool xhalf = false;
+  int hx_r1 = 0, hx_r2 = 0;
   size_t q_elems = 0;
   // host buffers of a run, kept with their pages between runs
   std::vector<host::PlanStep> plan_steps;
@@ -679,9 +688,20 @@ void launch_line(clat_engine* e, const void* in, void* out, long long bs, int ki
     a.strip = 0;
   }
   a.in_bs = a.out_bs = bs;
-  L.T2L = sizeof(TC) == 8 ? (const void*)e->dtw[0] : (const void*)e->dtw32[0];
+  // half-length lines: the Xi / 2-point lines' table W_Xi^m, m < Xi, the
+  // phase table W_2Xi^f, and the site cells halved (x = 2j + p -> j).  The
+  // FP32X forward keeps full lines (see xhalf)
+  const bool hl = e->xhalf && (MODE == PM_INV || e->rprec != CLAT_PREC_FP32X);
+  L.hl = hl;
+  if (hl) {
+    L.T2L = sizeof(TC) == 8 ? e->tw64[0] : e->tw32[0];
+    L.PH = sizeof(TC) == 8 ? (const void*)e->dtw[0] : (const void*)e->dtw32[0];
+  } else {
+    L.T2L = sizeof(TC) == 8 ? (const void*)e->dtw[0] : (const void*)e->dtw32[0];
+    L.PH = nullptr;
+  }
   constexpr int combo = combo_of<TI, TC, TO>();
-  const int R1 = e->cxf_s.R1, R2 = e->cxf_s.R2;
+  const int R1 = hl ? e->hx_r1 : e->cxf_s.R1, R2 = hl ? e->hx_r2 : e->cxf_s.R2;
   int lines = 0;
   switch (R1 * 64 + R2) {
 #define CLAT_L2_N(ra, rb) case ra * 64 + rb: lines = clat_l2_lines_##ra##x##rb(combo); break;
@@ -1589,7 +1609,7 @@ std::map<std::vector<int64_t>, int> g_tuned;
 std::vector<int64_t> tune_key(const clat_engine* e) {
   return {e->Xi, e->Yi, e->Zi, e->rprec, e->mix, e->sched, e->nb_max, e->n_live, e->n_sites,
           e->atile ? 1 : 0, e->btile ? 1 : 0, e->n_cu, e->coset ? e->sub->n_live : -1,
-          e->coset && e->sub->half ? 1 : 0};
+          e->coset && e->sub->half ? 1 : 0, e->coset && e->sub->xhalf ? 1 : 0};
 }
 
 // The fixed rule's prepares per launch: the engine's own, or on the
@@ -2071,6 +2091,11 @@ void set_half(clat_engine* s, bool half) {
   s->q_elems = (size_t)2 * s->Yi * s->gc_ncb * (2 * s->Zi) * C;
   s->a_set = s->d_set = (size_t)s->Zi * s->wcols * s->Yi;
   s->b_set = (size_t)2 * s->Yi * s->wcols * s->Zi;
+  int r1 = 0, r2 = 0;
+  s->xhalf = half && s->Xi % 2 == 0 && split2(s->Xi / 2, r1, r2) && col_pair_compiled(r1, r2);
+  if (const char* g = std::getenv("CLAT_HALFLINES")) s->xhalf = s->xhalf && std::atoi(g) != 0;
+  s->hx_r1 = s->xhalf ? r1 : 0;
+  s->hx_r2 = s->xhalf ? r2 : 0;
 }
 
 // Sites of the two-coset path: coset 0 (all coordinates even), then coset 1,
@@ -2649,12 +2674,17 @@ int clat_set_atoms(clat_engine* e, const int32_t* cell, int64_t n_atoms, int n_o
     const int yr1 = e->cyf.R1, yr2 = e->cyf.R2;
     const bool ymask = e->xline && yr2 <= 32;
     std::vector<uint32_t> mask(ymask ? (size_t)e->Zi * yr1 : 1, 0u);
+    // (on half-length lines the kernels take ccell >> 1 = rank Xi / 2 + j)
     for (int64_t l = 0; l < nlines; ++l) {
       const int32_t b = site_ptr[(size_t)l], n = site_ptr[(size_t)l + 1] - b;
       if (n == 0) continue;
       const int64_t rank = (int64_t)lines.size();
-      for (int32_t k = b; k < b + n; ++k)
-        ccell[(size_t)k] = (int32_t)(rank * e->Xi + (site_cell[(size_t)k] - l * e->Xi));
+      const int par = (int)((l % e->Yi + l / e->Yi) & 1);
+      for (int32_t k = b; k < b + n; ++k) {
+        const int x = (int)(site_cell[(size_t)k] - l * e->Xi);
+        if (e->xhalf && (x & 1) != par) throw host::Error(CLAT_EINVAL, "half-length lines: a site off x = y + z (mod 2)");
+        ccell[(size_t)k] = (int32_t)(rank * e->Xi + x);
+      }
       lines.push_back((int32_t)l);
       live_ptr.push_back(b + n);
       if (ymask) {
@@ -2891,6 +2921,7 @@ int clat_engine_kernels(clat_engine* e, uint32_t* flags) {
   if (e->atile) f |= CLAT_KERNELS_AD_STRIPS;
   if (e->coset) f |= CLAT_KERNELS_COSETS;
   if (e->coset && e->sub->half) f |= CLAT_KERNELS_HALF_BOX;
+  if (e->coset && e->sub->xhalf) f |= CLAT_KERNELS_HALF_LINES;
   *flags = f;
   return CLAT_OK;
 }

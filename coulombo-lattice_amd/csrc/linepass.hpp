@@ -11,6 +11,14 @@
 // Index map (L = R1 R2): n = n1 + R1 n2, k = R2 k1 + k2; the half-shift phase
 // h[n] = W_2L^n is split into W_2L^{n1 (2 k2 + s)} (table T2L of 2L roots)
 // and a compile-time rotation by W_2R2^n2, as in the column passes.
+//
+// Half-length lines (HL, table PH): a line of 2L points whose sites all have
+// x = 2j + p, p = (y + z) mod 2 (the fcc sublattice of the grid), is carried
+// as the L points j.  Its 2L frequencies below 2L of the 4L-point transform
+// are F(f) = W_4L^{f p} sum_j u_j W_2L^{f j}: the L-point line's transform
+// times a phase (table PH of 4L roots, W_4L^f), applied to the odd lines on
+// the way out of the forward pass and conjugated on the way into the
+// inverse one, whose first L outputs are then the potential at j.
 #pragma once
 
 #include "colpass.hpp"
@@ -23,7 +31,8 @@ struct LineArgs {
   int n_live;                   // occupied lines
   const int32_t* lines;         // line id of each compact index
   const int32_t* live_ptr;      // CSR of the sites over compact lines (n_live + 1)
-  const int32_t* site_ccell;    // compact cell of each site: compact line * L + x
+  const int32_t* site_ccell;    // compact cell of each site: compact line * L + x (half-length
+                                // lines: compact line * 2L + x, whose half is compact line * L + j)
   // batched prepares (blockIdx.y = b < LINE_NB): site arrays and dense-side offset of each
   const double2* site_in[LINE_NB];  // forward: pair density at the sites
   double2* site_out[LINE_NB];       // inverse: potential at the sites
@@ -48,6 +57,11 @@ struct LineArgs {
 __device__ __forceinline__ int line_base(const LineArgs& a, int line) {
   const int z = line / a.Yl;
   return z * a.lz + (line - z * a.Yl) * a.ly;
+}
+// (y + z) mod 2 of a line: the x parity of its sites on half-length lines
+__device__ __forceinline__ bool line_odd(const LineArgs& a, int line) {
+  const int z = line / a.Yl;
+  return ((line - z * a.Yl + z) & 1) != 0;
 }
 __device__ __forceinline__ int freq_off(const LineArgs& a, int f) {
   return a.lgsw ? (f >> a.lgsw) * a.strip + (f & ((1 << a.lgsw) - 1)) : f;
@@ -103,10 +117,11 @@ constexpr size_t line_lds() {
   return (size_t)C * NSUB * R2 * line_r1p<R1>() * sizeof(C2<TC>);
 }
 
-template <typename TI, typename TC, typename TO, int MODE, int R1, int R2, int NSUB, int C>
+template <typename TI, typename TC, typename TO, int MODE, int R1, int R2, int NSUB, int C, bool HL>
 __global__ __launch_bounds__((line_threads<C, R1, R2, NSUB>())) void k_linepass(const C2<TI>* in, C2<TO>* out,
                                                                                 LineArgs a,
-                                                                                const C2<TC>* __restrict__ T2L) {
+                                                                                const C2<TC>* __restrict__ T2L,
+                                                                                const C2<TC>* __restrict__ PH) {
   using CT = C2<TC>;
   constexpr int L = R1 * R2, R1P = line_r1p<R1>();
   constexpr int NTH = line_threads<C, R1, R2, NSUB>();
@@ -147,7 +162,7 @@ __global__ __launch_bounds__((line_threads<C, R1, R2, NSUB>())) void k_linepass(
     for (int k = s0 + tid; k < s1; k += NTH) {
       double2 v = ldg_stream(a.site_in[blockIdx.y] + k);
       if (in2) v.y = ldg_stream(in2 + k).x;
-      lds[a.site_ccell[k] - cell0] = mk<TC>((TC)v.x, (TC)v.y);
+      lds[(a.site_ccell[k] >> (HL ? 1 : 0)) - cell0] = mk<TC>((TC)v.x, (TC)v.y);
     }
     __syncthreads();
     CT v[R2];
@@ -189,10 +204,17 @@ __global__ __launch_bounds__((line_threads<C, R1, R2, NSUB>())) void k_linepass(
       rdft<R1, false, TC>(w);
       // block-uniform base of the prepare, 32-bit byte offsets per lane
       C2<TO>* ob = out + (size_t)blockIdx.y * a.out_bs;
-      const int lb = line_base(a, a.lines[b0 + c2]);
+      const int line = a.lines[b0 + c2];
+      const int lb = line_base(a, line);
       int off[R1];
       freq_offs<R1, R2 * NSUB>(a, k2 * NSUB + s2, off);
       const int f0 = k2 * NSUB + s2;
+      if constexpr (HL) {
+        if (line_odd(a, line)) {
+#pragma unroll
+          for (int k1 = 0; k1 < R1; ++k1) w[k1] = cmul<TC>(w[k1], PH[f0 + k1 * R2 * NSUB]);
+        }
+      }
 #pragma unroll
       for (int k1 = 0; k1 < R1; ++k1) {
         const uint32_t o = f0 + k1 * R2 * NSUB < a.fmax ? (uint32_t)(lb + off[k1]) * (uint32_t)sizeof(C2<TO>) : COL_OOB;
@@ -206,7 +228,8 @@ __global__ __launch_bounds__((line_threads<C, R1, R2, NSUB>())) void k_linepass(
     if (ok2) {
       CT w[R1];
       const C2<TI>* ib = in + (size_t)blockIdx.y * a.in_bs;
-      const int lb = line_base(a, a.lines[b0 + c2]);
+      const int line = a.lines[b0 + c2];
+      const int lb = line_base(a, line);
       int off[R1];
       freq_offs<R1, R2 * NSUB>(a, k2 * NSUB + s2, off);
       const int f0 = k2 * NSUB + s2;
@@ -214,6 +237,12 @@ __global__ __launch_bounds__((line_threads<C, R1, R2, NSUB>())) void k_linepass(
       for (int k1 = 0; k1 < R1; ++k1) {
         const uint32_t o = f0 + k1 * R2 * NSUB < a.fmax ? (uint32_t)(lb + off[k1]) * (uint32_t)sizeof(C2<TI>) : COL_OOB;
         w[k1] = cvt<TI, TC>(ldg_stream_off(ib, o));
+      }
+      if constexpr (HL) {
+        if (line_odd(a, line)) {
+#pragma unroll
+          for (int k1 = 0; k1 < R1; ++k1) w[k1] = cmulc<TC>(w[k1], PH[f0 + k1 * R2 * NSUB]);
+        }
       }
       rdft<R1, true, TC>(w);
       const uint32_t tstep = (uint32_t)(2 * k2 + s2) * SC;
@@ -258,7 +287,7 @@ __global__ __launch_bounds__((line_threads<C, R1, R2, NSUB>())) void k_linepass(
     // ---------------------------------------------------------- gather
     double2* out2 = a.site_out2[blockIdx.y];
     for (int k = s0 + tid; k < s1; k += NTH) {
-      const int cc = a.site_ccell[k] - cell0;
+      const int cc = (a.site_ccell[k] >> (HL ? 1 : 0)) - cell0;
       const int c = cc / L, x = cc - c * L;
       const CT u = lds[(c * NSUB) * L + x];
       double2 r = make_double2((double)u.x, (double)u.y);
@@ -284,6 +313,8 @@ struct LineLaunch {
   void* out;
   LineArgs a;
   const void* T2L;
+  const void* PH;  // half-length lines: W_4L^f, f < 2L (else null)
+  bool hl;         // half-length lines
 };
 
 template <typename TI, typename TC, typename TO, int MODE, int R1, int R2>
@@ -293,8 +324,14 @@ hipError_t line_go(const LineLaunch& L) {
   constexpr int nth = line_threads<C, R1, R2, NSUB>();
   constexpr size_t lds = line_lds<TC, R1, R2, NSUB, C>();
   static_assert(lds <= 65536, "line pass tile exceeds 64 KiB of LDS");
-  hipLaunchKernelGGL((k_linepass<TI, TC, TO, MODE, R1, R2, NSUB, C>), dim3(L.blocks, L.nb), dim3(nth), lds, L.stream,
-                     (const C2<TI>*)L.in, (C2<TO>*)L.out, L.a, (const C2<TC>*)L.T2L);
+  if (L.hl)
+    hipLaunchKernelGGL((k_linepass<TI, TC, TO, MODE, R1, R2, NSUB, C, true>), dim3(L.blocks, L.nb), dim3(nth), lds,
+                       L.stream, (const C2<TI>*)L.in, (C2<TO>*)L.out, L.a, (const C2<TC>*)L.T2L,
+                       (const C2<TC>*)L.PH);
+  else
+    hipLaunchKernelGGL((k_linepass<TI, TC, TO, MODE, R1, R2, NSUB, C, false>), dim3(L.blocks, L.nb), dim3(nth), lds,
+                       L.stream, (const C2<TI>*)L.in, (C2<TO>*)L.out, L.a, (const C2<TC>*)L.T2L,
+                       (const C2<TC>*)L.PH);
   return hipGetLastError();
 }
 

@@ -272,7 +272,10 @@ int clat_engine_info(clat_engine* e, int dim[3], int64_t* device_bytes);
  * environment keeps the single-grid path), and its half-box form (every
  * coarse position on mx + my + mz even, as zincblende's are: only the x
  * frequencies below half the coarse box are carried, against kernels summed
- * over the half-box shift; CLAT_HALFBOX=0 keeps all of them).  An
+ * over the half-box shift; CLAT_HALFBOX=0 keeps all of them), and in the
+ * half box the x passes on half-length lines (a coarse line's sites all have
+ * x = y + z (mod 2): the line is carried as its Xc / 2 sites and the x
+ * parity enters as a phase; CLAT_HALFLINES=0 keeps full lines).  An
  * introspection hook with no counterpart in the reference: every combination
  * gives the same results. */
 #define CLAT_KERNELS_TWO_LEVEL 1u
@@ -283,6 +286,7 @@ int clat_engine_info(clat_engine* e, int dim[3], int64_t* device_bytes);
 #define CLAT_KERNELS_AD_STRIPS 32u
 #define CLAT_KERNELS_COSETS 64u
 #define CLAT_KERNELS_HALF_BOX 128u
+#define CLAT_KERNELS_HALF_LINES 256u
 int clat_engine_kernels(clat_engine* e, uint32_t* flags);
 
 /* ---- planning ---------------------------------------------------------- */

@@ -134,6 +134,8 @@ def test_one_million_atoms_full_size():
     vals = {}
     for prec in (clat.PREC_FP32X, clat.PREC_FP64):
         eng = _engine(w, prec, 20)
+        # the path the bench measures: two cosets, half box, half-length x lines
+        assert {"cosets", "half_box", "half_lines"} <= eng.kernels()
         vals[prec] = eng.compute_integrals(quads)
         eng.close()
     _check_symmetries(quads, vals[clat.PREC_FP64], 1e-12)

@@ -228,3 +228,41 @@ def test_cosets_full_box(prec, monkeypatch):
     else:
         check_fp32x_model(gf, want, quads, "full box fp32x")
         check_fp32x_model(gh, want, quads, "half box fp32x")
+
+
+@pytest.mark.parametrize("prec", PRECS, ids=PIDS)
+@pytest.mark.parametrize("box", [(32, 16, 16), (40, 16, 16)])
+def test_half_lines_match_oracle(prec, box, monkeypatch):
+    """128x64x64 and 160x64x64 zincblende grids: coarse lines of 64 and 80
+    points, whose sites all have x = y + z (mod 2), carried as 32 and 40
+    points (pairs 8x4 and 8x5) with the x parity as a phase.  Integrals
+    against the oracle and against full coarse lines (CLAT_HALFLINES=0), the
+    per-atom potentials against full lines too."""
+    nh, ne = 2, 2
+    w = zb.workload(*box, nh, ne, config=0, sigma_frac=0.2, n_orb=4)
+    quads = clat.pattern_enumerate("****", nh + ne, nh)
+    print("case half lines", box, prec, w["dims"], flush=True)
+    monkeypatch.delenv("CLAT_HALFLINES", raising=False)
+    a = _engine(w, w["cells"], prec, 4, w["states"], True, monkeypatch)
+    monkeypatch.setenv("CLAT_HALFLINES", "0")
+    b = _engine(w, w["cells"], prec, 4, w["states"], True, monkeypatch)
+    monkeypatch.delenv("CLAT_HALFLINES")
+    assert {"cosets", "half_box", "half_lines"} <= a.kernels()
+    assert {"cosets", "half_box"} <= b.kernels() and "half_lines" not in b.kernels()
+    ga, gb = a.compute_integrals(quads), b.compute_integrals(quads)
+    pa, pb = a.potential_at_atoms(3), b.potential_at_atoms(3)
+    a.close()
+    b.close()
+    want = _oracle(w, w["cells"], w["states"], quads, nh + ne)
+    if prec == clat.PREC_FP64:
+        check(ga, want, prec, quads)
+        assert np.max(rel_err(ga, gb)) < 1e-12
+        assert np.max(np.abs(pa - pb)) < 1e-12 * np.max(np.abs(pb))
+    elif prec == clat.PREC_FP32X:
+        check_fp32x_model(ga, want, quads, "half lines fp32x")
+        check_fp32x_model(gb, want, quads, "full lines fp32x")
+        assert np.max(np.abs(pa - pb)) < 1e-6 * np.max(np.abs(pb))
+    else:
+        check(ga, want, prec, quads)
+        check(gb, want, prec, quads)
+
