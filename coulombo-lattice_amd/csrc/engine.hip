@@ -316,6 +316,7 @@ struct clat_engine {
   bool coset = false;           // the current atoms run on the two-coset path
   int64_t n_u = 0;
   bool fine_pending = false;    // G64 holds the real-space table: its DCT-I and conv table wait for first use
+  bool g64_real = false;        // with fine_pending: G64 filled (else the table is gdefer's, not yet sampled)
   // kept by the coarse engine of a two-coset path
   clat_engine* kparent = nullptr;  // kernel timing goes to this engine's events
   bool own_stream = true;
@@ -1143,9 +1144,9 @@ void dct_axis(clat_engine* e, int axis, double scale) {
   e->kend(K_DCT, 16.0 * (double)e->NG());
 }
 
-// DCT-II (REDFT10) of the lines of e->Gt64 along one axis (M inputs, M + 1
-// outputs per line in the octant's pitch), through the same 2M-point plans
-void dct2_axis(clat_engine* e, int axis, double scale) {
+// DCT-I of e->G64 and DCT-II of e->Gt64 along one axis in one launch (the
+// two-coset tables; same line layout as dct_axis)
+void dct12_axis(clat_engine* e, int axis, double scale) {
   const int X1 = e->Xi + 1, Y1 = e->Yi + 1, Z1 = e->Zi + 1;
   const AxisPlan& p = axis == 0 ? e->dx : (axis == 1 ? e->dy : e->dz);
   const int L2 = p.L;
@@ -1153,20 +1154,20 @@ void dct2_axis(clat_engine* e, int axis, double scale) {
   const size_t lds = (size_t)C * L2 * sizeof(double2);
   e->kbegin(K_DCT);
   if (axis == 0) {
-    const int ncols = Y1 * Z1;
-    hipLaunchKernelGGL((k_dct2<16, true>), dim3((ncols + 2 * C - 1) / (2 * C)), dim3(NT), lds, e->stream, e->Gt64,
-                       p, e->dtw[0], C, log2i(C), 0LL, 1LL, (long long)X1, ncols, scale);
+    const int ncols = Y1 * Z1, nblk = (ncols + 2 * C - 1) / (2 * C);
+    hipLaunchKernelGGL((k_dct12<16, true>), dim3(2 * nblk), dim3(NT), lds, e->stream, e->G64, e->Gt64, p, e->dtw[0],
+                       nblk, C, log2i(C), 0LL, 1LL, (long long)X1, ncols, scale);
   } else if (axis == 1) {
-    const int ncols = X1;
-    hipLaunchKernelGGL((k_dct2<16, false>), dim3(((ncols + 2 * C - 1) / (2 * C)) * Z1), dim3(NT), lds, e->stream,
-                       e->Gt64, p, e->dtw[1], C, log2i(C), (long long)X1 * Y1, (long long)X1, 1LL, ncols, scale);
+    const int ncols = X1, nblk = ((ncols + 2 * C - 1) / (2 * C)) * Z1;
+    hipLaunchKernelGGL((k_dct12<16, false>), dim3(2 * nblk), dim3(NT), lds, e->stream, e->G64, e->Gt64, p,
+                       e->dtw[1], nblk, C, log2i(C), (long long)X1 * Y1, (long long)X1, 1LL, ncols, scale);
   } else {
-    const int ncols = X1 * Y1;
-    hipLaunchKernelGGL((k_dct2<16, false>), dim3((ncols + 2 * C - 1) / (2 * C)), dim3(NT), lds, e->stream,
-                       e->Gt64, p, e->dtw[2], C, log2i(C), 0LL, (long long)X1 * Y1, 1LL, ncols, scale);
+    const int ncols = X1 * Y1, nblk = (ncols + 2 * C - 1) / (2 * C);
+    hipLaunchKernelGGL((k_dct12<16, false>), dim3(2 * nblk), dim3(NT), lds, e->stream, e->G64, e->Gt64, p,
+                       e->dtw[2], nblk, C, log2i(C), 0LL, (long long)X1 * Y1, 1LL, ncols, scale);
   }
   LAUNCH_CHECK();
-  e->kend(K_DCT, 16.0 * (double)e->NG());
+  e->kend(K_DCT, 32.0 * (double)e->NG());
 }
 
 void tile_conv_table(clat_engine* e);
@@ -1183,16 +1184,17 @@ void build_coset_tables(clat_engine* e) {
   clat_engine* s = e->sub;
   s->stream = e->stream;
   if (e->fine_pending) {
-    hipLaunchKernelGGL(k_coset_samples, dim3(grid1d(s->NG())), dim3(256), 0, e->stream, e->G64, e->Xi, e->Yi,
-                       e->Zi, s->G64, s->Gt64);
+    if (e->g64_real)
+      hipLaunchKernelGGL(k_coset_samples, dim3(grid1d(s->NG())), dim3(256), 0, e->stream, e->G64, e->Xi, e->Yi,
+                         e->Zi, s->G64, s->Gt64);
+    else
+      hipLaunchKernelGGL(k_coset_gtable, dim3(grid1d(s->NG())), dim3(256), 0, e->stream, e->gdefer, e->Xi, e->Yi,
+                         e->Zi, s->G64, s->Gt64);
     LAUNCH_CHECK();
     const double sc = 1.0 / (8.0 * (double)s->Xi * (double)s->Yi * (double)s->Zi);
-    dct_axis(s, 0, 1.0);
-    dct_axis(s, 1, 1.0);
-    dct_axis(s, 2, sc);
-    dct2_axis(s, 0, 1.0);
-    dct2_axis(s, 1, 1.0);
-    dct2_axis(s, 2, sc);
+    dct12_axis(s, 0, 1.0);
+    dct12_axis(s, 1, 1.0);
+    dct12_axis(s, 2, sc);
   } else {
     hipLaunchKernelGGL(k_coarse_g0, dim3(grid1d(s->NG())), dim3(256), 0, e->stream, e->G64, e->Xi, e->Yi, e->Zi,
                        s->G64);
@@ -1231,6 +1233,11 @@ void build_coset_tables(clat_engine* e) {
 // the fine table's DCT-I (and FP32 copy) and conv table, once per interaction
 void ensure_fine_tables(clat_engine* e) {
   if (!e->fine_pending) return;
+  if (!e->g64_real) {  // the interaction's table, not built on the two-coset path
+    hipLaunchKernelGGL(k_gtable, dim3(grid1d(e->NG())), dim3(256), 0, e->stream, e->G64, e->gdefer);
+    LAUNCH_CHECK();
+    e->g64_real = true;
+  }
   dct_axis(e, 0, 1.0);
   dct_axis(e, 1, 1.0);
   dct_axis(e, 2, 1.0 / (8.0 * (double)e->Xi * (double)e->Yi * (double)e->Zi));
@@ -1297,8 +1304,9 @@ void build_table(clat_engine* e, GArgs g) {
 bool ensure_table(clat_engine* e) {
   if (!e->table_deferred) return false;
   HIPX(hipEventRecord(e->stage_ev[0], e->stream));
-  hipLaunchKernelGGL(k_gtable, dim3(grid1d(e->NG())), dim3(256), 0, e->stream, e->G64, e->gdefer);
-  LAUNCH_CHECK();
+  // the fine real-space table is built only where a fine-grid pass needs it
+  // (the two-coset path samples G(2d) and G(2d + 1) from the arguments)
+  e->g64_real = false;
   finish_interaction(e);
   e->table_deferred = false;
   HIPX(hipEventRecord(e->stage_ev[5], e->stream));
@@ -2544,6 +2552,7 @@ int clat_set_interaction_table(clat_engine* e, const double* G) {
     e->dfree(tmp, nu * sizeof(double));
     HIPX(ce);
     HIPX(se);
+    e->g64_real = true;
     finish_interaction(e);
     HIPX(hipStreamSynchronize(e->stream));
     e->kcollect();

@@ -263,6 +263,19 @@ __device__ __forceinline__ double eps_of_r(const GArgs& g, double r) {
   return g.dielectric;
 }
 
+// G at fine offset (ix, iy, iz) of the octant (zero past the user's table)
+__device__ __forceinline__ double gval(const GArgs& g, int ix, int iy, int iz) {
+  double v = 0.0;
+  if (ix < g.ux && iy < g.uy && iz < g.uz) {
+    const double z = iz * g.sz, z2 = z * z;
+    const double y = iy * g.sy, y2z2 = z2 + y * y;
+    const double x = ix * g.sx, r2 = y2z2 + x * x, r = sqrt(r2);
+    v = 14.39963737103201 / (eps_of_r(g, r) * r);
+    if (ix == 0 && iy == 0 && iz == 0) v = g.onsite;
+  }
+  return v;
+}
+
 __global__ void k_gtable(double* __restrict__ G, GArgs g) {
   const size_t n = (size_t)g.X1 * g.Y1 * g.Z1;
   for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < n; t += (size_t)gridDim.x * blockDim.x) {
@@ -270,15 +283,7 @@ __global__ void k_gtable(double* __restrict__ G, GArgs g) {
     const size_t q = t / g.X1;
     const int iy = (int)(q % g.Y1);
     const int iz = (int)(q / g.Y1);
-    double v = 0.0;
-    if (ix < g.ux && iy < g.uy && iz < g.uz) {
-      const double z = iz * g.sz, z2 = z * z;
-      const double y = iy * g.sy, y2z2 = z2 + y * y;
-      const double x = ix * g.sx, r2 = y2z2 + x * x, r = sqrt(r2);
-      v = 14.39963737103201 / (eps_of_r(g, r) * r);
-      if (ix == 0 && iy == 0 && iz == 0) v = g.onsite;
-    }
-    G[t] = v;
+    G[t] = gval(g, ix, iy, iz);
   }
 }
 
@@ -301,21 +306,18 @@ __global__ void k_gpad(const double* __restrict__ src, double* __restrict__ G, G
 // C lines.  The tile is index-major when cs == 1 (consecutive columns) and
 // line-major when is == 1.  The result is scaled by `scale`.
 template <int EMAX, bool LM>
-__global__ __launch_bounds__(NT) void k_dct1(double* __restrict__ G, AxisPlan p,
-                                             const double2* __restrict__ tw, int nlines, int C, int lgC,
-                                             long long os, long long is, long long cs, int ncols,
-                                             double scale) {
+__device__ __forceinline__ void dct1_block(int bid, double2* buf, double* __restrict__ G, const AxisPlan& p,
+                                           const double2* __restrict__ tw, int C, int lgC, long long os,
+                                           long long is, long long cs, int ncols, double scale) {
   // Two real columns per complex line: the even extensions of columns c and
   // c + C are real and even, so their transforms are real and come back as
   // the real and imaginary parts of one complex FFT.
-  extern __shared__ __align__(16) unsigned char smem_raw[];
-  double2* buf = reinterpret_cast<double2*>(smem_raw);
   const int L2 = p.L;      // 2M
   const int M = L2 / 2;
   // block -> (outer o, first column c0); 2C columns per block
   const int ncb = (ncols + 2 * C - 1) / (2 * C);
-  const int o = blockIdx.x / ncb;
-  const int c0 = (blockIdx.x - o * ncb) * 2 * C;
+  const int o = bid / ncb;
+  const int c0 = (bid - o * ncb) * 2 * C;
   const int nC = min(2 * C, ncols - c0);
   double* base = G + (size_t)o * os + (size_t)c0 * cs;
   const int tot = C * L2;
@@ -355,6 +357,16 @@ __global__ __launch_bounds__(NT) void k_dct1(double* __restrict__ G, AxisPlan p,
     if (c < nC) base[(size_t)k * is + (size_t)c * cs] = r.x * scale;
     if (c + C < nC) base[(size_t)k * is + (size_t)(c + C) * cs] = r.y * scale;
   }
+}
+
+template <int EMAX, bool LM>
+__global__ __launch_bounds__(NT) void k_dct1(double* __restrict__ G, AxisPlan p,
+                                             const double2* __restrict__ tw, int nlines, int C, int lgC,
+                                             long long os, long long is, long long cs, int ncols,
+                                             double scale) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  dct1_block<EMAX, LM>((int)blockIdx.x, reinterpret_cast<double2*>(smem_raw), G, p, tw, C, lgC, os, is, cs, ncols,
+                       scale);
   (void)nlines;
 }
 
@@ -364,20 +376,17 @@ __global__ __launch_bounds__(NT) void k_dct1(double* __restrict__ G, AxisPlan p,
 //   Y_k = 2 Re(e^{-i pi k / (2M)} X_k).
 // Two real columns c and c + C share one complex line (the real and
 // imaginary parts); their transforms are separated from X_k and X_{2M-k}.
-// Line layout and launch shape as k_dct1; a line holds M inputs and M + 1
+// Line layout and launch shape as k_dct1 (run by k_dct12); a line holds M inputs and M + 1
 // outputs in place (the table's pitch has room for the extra one).
 template <int EMAX, bool LM>
-__global__ __launch_bounds__(NT) void k_dct2(double* __restrict__ G, AxisPlan p,
-                                             const double2* __restrict__ tw, int C, int lgC,
-                                             long long os, long long is, long long cs, int ncols,
-                                             double scale) {
-  extern __shared__ __align__(16) unsigned char smem_raw[];
-  double2* buf = reinterpret_cast<double2*>(smem_raw);
+__device__ __forceinline__ void dct2_block(int bid, double2* buf, double* __restrict__ G, const AxisPlan& p,
+                                           const double2* __restrict__ tw, int C, int lgC, long long os,
+                                           long long is, long long cs, int ncols, double scale) {
   const int L2 = p.L;  // 2M
   const int M = L2 / 2;
   const int ncb = (ncols + 2 * C - 1) / (2 * C);
-  const int o = blockIdx.x / ncb;
-  const int c0 = (blockIdx.x - o * ncb) * 2 * C;
+  const int o = bid / ncb;
+  const int c0 = (bid - o * ncb) * 2 * C;
   const int nC = min(2 * C, ncols - c0);
   double* base = G + (size_t)o * os + (size_t)c0 * cs;
   const int tot = C * L2;
@@ -411,6 +420,21 @@ __global__ __launch_bounds__(NT) void k_dct2(double* __restrict__ G, AxisPlan p,
   }
 }
 
+// The two-coset tables' transforms along one axis in one launch: DCT-I of
+// the lines of G0 in blocks [0, nblk), DCT-II of the lines of Gt (same
+// shape) in blocks [nblk, 2 nblk).  Each alone is a launch of about one wave
+// per SIMD on the 1M-atom workload's coarse octant.
+template <int EMAX, bool LM>
+__global__ __launch_bounds__(NT) void k_dct12(double* __restrict__ G0, double* __restrict__ Gt, AxisPlan p,
+                                              const double2* __restrict__ tw, int nblk, int C, int lgC,
+                                              long long os, long long is, long long cs, int ncols, double scale) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  double2* buf = reinterpret_cast<double2*>(smem_raw);
+  const int b = (int)blockIdx.x;
+  if (b < nblk) dct1_block<EMAX, LM>(b, buf, G0, p, tw, C, lgC, os, is, cs, ncols, scale);
+  else dct2_block<EMAX, LM>(b - nblk, buf, Gt, p, tw, C, lgC, os, is, cs, ncols, scale);
+}
+
 // Real-space samples of the two-coset kernels from the fine real-space octant
 // G ((Zi+1)(Yi+1)(Xi+1), G at fine offset (ix, iy, iz)): G0 = G(2d) on the
 // coarse octant d in [0, Xc]^3, Gt = G(2d + (1,1,1)) on d in [0, Xc)^3 in a
@@ -427,6 +451,21 @@ __global__ void k_coset_samples(const double* __restrict__ G, int Xi, int Yi, in
     const int dz = (int)(u / (Yc + 1));
     G0[t] = G[((size_t)(2 * dz) * Y1 + 2 * dy) * X1 + 2 * dx];
     Gt[t] = (dx < Xc && dy < Yc && dz < Zc) ? G[((size_t)(2 * dz + 1) * Y1 + 2 * dy + 1) * X1 + 2 * dx + 1] : 0.0;
+  }
+}
+
+// The same samples straight from the interaction's arguments (no fine table):
+// G0 = G(2d), Gt = G(2d + (1,1,1)), bit-identical to k_gtable's entries.
+__global__ void k_coset_gtable(GArgs g, int Xi, int Yi, int Zi, double* __restrict__ G0, double* __restrict__ Gt) {
+  const int Xc = Xi / 2, Yc = Yi / 2, Zc = Zi / 2;
+  const size_t n = (size_t)(Xc + 1) * (Yc + 1) * (Zc + 1);
+  for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < n; t += (size_t)gridDim.x * blockDim.x) {
+    const int dx = (int)(t % (Xc + 1));
+    const size_t u = t / (Xc + 1);
+    const int dy = (int)(u % (Yc + 1));
+    const int dz = (int)(u / (Yc + 1));
+    G0[t] = gval(g, 2 * dx, 2 * dy, 2 * dz);
+    Gt[t] = (dx < Xc && dy < Yc && dz < Zc) ? gval(g, 2 * dx + 1, 2 * dy + 1, 2 * dz + 1) : 0.0;
   }
 }
 

@@ -42,7 +42,8 @@ def kind_of(name):
         return {0: "xfwd", 2: "xinv"}[int(m.group(4))]
     for key in ("k_products_direct", "k_products", "k_overlap_partial", "k_overlap_reduce", "k_dct1",
                 "k_gconv", "k_gtable", "k_gpad", "k_d2f", "k_xfwd", "k_xinv", "k_col", "k_grid_dot",
-                "k_atoms_from_sites", "k_coarse_g0", "k_coarse_gt", "copyBuffer"):
+                "k_atoms_from_sites", "k_coarse_g0", "k_coarse_gt", "k_coarse_q", "k_coset_gtable",
+                "k_coset_samples", "k_coeffs_to_sites", "copyBuffer"):
         if key in name:
             return key[2:] if key.startswith("k_") else key
     return name.split("(")[0][:60]

@@ -11,8 +11,8 @@ with G_p(d) = G(2d + p) convolved over the coarse grid.  The engine builds the
 coarse transforms of G_0 and G_(1,1,1) in one of two ways: from the fine DCT-I
 table Gs (the octant of DCT-I(G) / (8XYZ), CoulombCalculator.cpp:57-103) by
 summing its eight half-box shifts (kernels.hpp: k_coarse_g0, k_coarse_gt), or
-from the real-space samples through a coarse DCT-I and DCT-II (k_coset_samples,
-k_dct2, k_coarse_q).  These tests pin both against the transforms of the
+from the real-space samples through a coarse DCT-I and DCT-II (k_coset_gtable or k_coset_samples,
+k_dct12, k_coarse_q).  These tests pin both against the transforms of the
 sampled kernels, and the whole decomposition against the oracle's eight-round
 prepare (CoulombCalculator.cpp:105-163)."""
 import numpy as np
@@ -73,7 +73,7 @@ def test_coset_tables_are_coarse_transforms(dims, tf):
 @pytest.mark.parametrize("dims", [(12, 8, 10), (8, 14, 6)])
 def test_coset_tables_from_real_space_samples(dims):
     """The second route to the same tables, from the real-space fine table
-    before its DCT (kernels.hpp: k_coset_samples, k_dct2, k_coarse_q): the
+    before its DCT (kernels.hpp: k_coset_gtable, k_dct12, k_coarse_q): the
     within-coset octant is DCT-I(G(2d)) / (8 N_coarse) on d in [0, X/2]^3, and
     the cross-coset one e^{i pi q.(1/X)} DCT-II(G(2d + 1)) / (8 N_coarse) on d in
     [0, X/2)^3, odd about X/2."""
