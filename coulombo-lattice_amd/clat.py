@@ -384,13 +384,13 @@ class Engine:
         _check(lib().clat_set_tuning(self.handle, 1 if measure else 0))
 
     def tuning(self) -> dict:
-        """Prepares per launch in use, measured ms of 1/2/4/8, graph counts."""
+        """Prepares per launch in use, measured ms per tried count, graph counts."""
         b = ctypes.c_int()
-        ms = np.zeros(4)
+        ms = np.zeros(8)
         rep = _i64()
         cap = _i64()
         _check(lib().clat_tuning_info(self.handle, ctypes.byref(b), ms, ctypes.byref(rep), ctypes.byref(cap)))
-        return {"batch": b.value, "trial_ms": {1 << i: float(ms[i]) for i in range(4) if ms[i] > 0},
+        return {"batch": b.value, "trial_ms": {i + 1: float(ms[i]) for i in range(8) if ms[i] > 0},
                 "graph_replays": rep.value, "graph_captures": cap.value}
 
     def info(self):

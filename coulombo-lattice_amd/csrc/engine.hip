@@ -291,7 +291,7 @@ struct clat_engine {
   int nb_rule = 1;
   int tune = 1;                         // 1: measure (the default), 0: rule only
   int tuned_nb = 0;                     // 0: not measured for the current atoms
-  double tune_ms[4] = {0, 0, 0, 0};     // prepare-stage ms of batches 1, 2, 4, 8 (0: not tried)
+  double tune_ms[8] = {};               // prepare-stage ms of k + 1 prepares per launch (0: not tried)
   hipEvent_t tune_ev[9] = {};
   int64_t atoms_gen = 0;                // bumped by every clat_set_atoms
   // the device stage of a run (prepares, overlaps) replayed as one HIP graph
@@ -1621,13 +1621,19 @@ std::vector<int64_t> tune_key(const clat_engine* e) {
 }
 
 // The fixed rule's prepares per launch: the engine's own, or on the
-// two-coset path eight where sixteen coarse B planes take at most 1.1 GB
-// (the 1M-atom grid's coset planes: 2.88 ms per prepare stage with eight
-// against 3.00 with four)
+// two-coset path as many as keep a launch's B planes (both cosets) within 3/4
+// of the 256 MiB Infinity Cache, so that the conv reads what the y forward
+// just wrote from it (1M-atom step, 32 MiB of B per prepare: conv 0.45 ms
+// per step at six, 0.51 at seven, 0.57 at eight; step 2.84, 2.86 and 2.87
+// ms on one box); where fewer than four fit, eight where sixteen coarse B
+// planes take at most 1.1 GB, else four
+constexpr size_t MALL_BYTES = (size_t)256 << 20;
 int batch_rule(const clat_engine* e) {
   if (!e->coset || std::getenv("CLAT_BATCH")) return e->nb_rule;
-  const size_t planes = 16 * e->sub->b_set * e->sub->s_b;
-  return std::min(e->nb_max, planes <= ((size_t)1100 << 20) ? 8 : 4);
+  const size_t per = 2 * e->sub->b_set * e->sub->s_b;
+  const int fit = (int)std::min<size_t>(8, MALL_BYTES / 4 * 3 / std::max<size_t>(1, per));
+  if (fit >= 4) return std::min(e->nb_max, fit);
+  return std::min(e->nb_max, 8 * per <= ((size_t)1100 << 20) ? 8 : 4);
 }
 
 // Prepares per pass launch for this run (the autotuner, the counterpart of
@@ -1648,9 +1654,13 @@ int choose_batch(clat_engine* e, const Units& units, const std::vector<int>& slo
     if (it != g_tuned.end()) return e->tuned_nb = it->second;
   }
   const int nU = (int)units.size();
+  // 1, 2, 4, 8 and the rule's count (up to the units of the run)
+  const int rule0 = batch_rule(e);
   std::vector<int> cand;
-  for (int c = 1; c <= e->nb_max; c *= 2)
+  for (int c = 1; c <= e->nb_max; c *= 2) {
+    if (rule0 > c / 2 && rule0 < c && rule0 / 2 < nU) cand.push_back(rule0);
     if (c == 1 || c / 2 < nU) cand.push_back(c);
+  }
   if (cand.size() == 1) return e->tuned_nb = cand[0];
   for (auto& ev : e->tune_ev)
     if (!ev) HIPX(hipEventCreate(&ev));
@@ -1681,11 +1691,11 @@ int choose_batch(clat_engine* e, const Units& units, const std::vector<int>& slo
   }
   e->kmask = kmask;
   size_t pick = 0;
+  for (double& t : e->tune_ms) t = 0.0;
   for (size_t i = 0; i < cand.size(); ++i) {
     if (best[i] < best[pick]) pick = i;
-    e->tune_ms[i] = best[i];
+    e->tune_ms[cand[i] - 1] = best[i];
   }
-  for (size_t i = cand.size(); i < 4; ++i) e->tune_ms[i] = 0.0;
   // the rule's count stays unless another is more than 2 % faster (the trials
   // of neighbouring counts often differ by less than their run-to-run noise)
   const int rule = batch_rule(e);
@@ -2941,12 +2951,12 @@ int clat_set_tuning(clat_engine* e, int measure) {
   return CLAT_OK;
 }
 
-int clat_tuning_info(clat_engine* e, int* batch, double trial_ms[4], int64_t* graph_replays,
+int clat_tuning_info(clat_engine* e, int* batch, double trial_ms[8], int64_t* graph_replays,
                      int64_t* graph_captures) {
   if (!e) return set_err(CLAT_EINVAL, "invalid arguments");
   if (batch) *batch = e->tune ? e->tuned_nb : batch_rule(e);
   if (trial_ms)
-    for (int i = 0; i < 4; ++i) trial_ms[i] = e->tune_ms[i];
+    for (int i = 0; i < 8; ++i) trial_ms[i] = e->tune_ms[i];
   if (graph_replays) *graph_replays = e->graph_replays;
   if (graph_captures) *graph_captures = e->graph_captures;
   return CLAT_OK;

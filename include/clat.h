@@ -295,23 +295,28 @@ int clat_engine_kernels(clat_engine* e, uint32_t* flags);
  * measureFFT = true) (CoulombCalculator.hpp:57, CoulombCalculator.cpp:23,31),
  * for this engine's own plans.  measure = 1 (the default, as in the
  * reference): the first clat_compute_integrals of an atom list times the
- * whole prepare sequence with 1, 2, 4 and 8 prepares per pass launch and
- * keeps the fastest, for the engine and for every later engine of the process
- * with the same grid, arithmetic and occupied lines.  The counts give
- * bit-identical results; the first run takes a few prepare sequences longer.
- * measure = 0: a fixed rule (four, or eight on grids whose eight
- * intermediate planes take at most 1.1 GB).  CLAT_TUNE=0 in the environment
- * at clat_create turns measuring off; CLAT_BATCH=n fixes the count. */
+ * whole prepare sequence with 1, 2, 4 and 8 prepares per pass launch and the
+ * fixed rule's count, and keeps the fastest (the rule's count where it is
+ * within 2 % of the fastest), for the engine and for every later engine of
+ * the process with the same grid, arithmetic and occupied lines.  The counts
+ * give bit-identical results; the first run takes a few prepare sequences
+ * longer.  measure = 0: the fixed rule (four, or eight on grids whose eight
+ * intermediate planes take at most 1.1 GB; on the two-coset path as many as
+ * keep a launch's intermediate planes within 192 MiB, three quarters of the
+ * device's 256 MiB Infinity Cache, where four or more fit).  CLAT_TUNE=0 in the
+ * environment at clat_create turns measuring off; CLAT_BATCH=n fixes the
+ * count. */
 int clat_set_tuning(clat_engine* e, int measure);
 
 /* The prepares per launch in use (0: not measured yet), the measured
- * prepare-sequence milliseconds of 1, 2, 4 and 8 (0: not tried), and how
+ * prepare-sequence milliseconds of k prepares per launch in trial_ms[k - 1],
+ * k = 1..8 (0: not tried), and how
  * often the device stage of clat_compute_integrals was captured into and
  * replayed from HIP graphs (runs with an unchanged plan and buffers replay
  * two graphs, prepares and overlaps, instead of launching every pass; runs
  * with kernel timing on, and every run after CLAT_GRAPH=0 at clat_create,
  * launch directly).  No counterpart in the reference. */
-int clat_tuning_info(clat_engine* e, int* batch, double trial_ms[4], int64_t* graph_replays,
+int clat_tuning_info(clat_engine* e, int* batch, double trial_ms[8], int64_t* graph_replays,
                      int64_t* graph_captures);
 
 #ifdef __cplusplus

@@ -94,7 +94,7 @@ def test_measured_batch_is_cached_and_bit_identical(monkeypatch):
     a = _engine(w, clat.PREC_FP32X)
     va = a.compute_integrals(quads)
     ta = a.tuning()
-    assert ta["batch"] in (1, 2, 4, 8) and len(ta["trial_ms"]) >= 2, ta
+    assert ta["batch"] in ta["trial_ms"] and len(ta["trial_ms"]) >= 2, ta
     b = _engine(w, clat.PREC_FP32X)
     vb = b.compute_integrals(quads)
     tb = b.tuning()
