@@ -176,3 +176,47 @@ def test_group_matches_per_engine_runs(monkeypatch):
     assert mixed[0].tuning()["graph_replays"] >= 2
     for e in mixed + solo:
         e.close()
+
+
+@pytest.mark.parametrize("box", [(8, 8, 8), (16, 16, 16)])
+def test_interaction_table_built_on_first_use(box):
+    """The interaction table is enqueued by the first call that needs it.
+    A second interaction set before any run replaces the first unseen; the
+    table built inside a potential run, inside an integral run or after a
+    user table gives the same bits as an engine that only ever saw the last
+    interaction (32^3 grid on the single grid, 64^3 on the two-coset path)."""
+    print("case deferred table", box, flush=True)
+    w = zb.workload(*box, 2, 2, config=0, sigma_frac=0.2, n_orb=4)
+    X, Y, Z = w["dims"]
+    quads = clat.pattern_enumerate("****", 4, 2)
+    first = clat.InteractionSimple(w["step"], 0.3, 4.0)
+    last = clat.InteractionSimple(w["step"], 0.0, 10.89)
+
+    def engine(inters):
+        eng = clat.Engine((X, Y, Z), clat.PREC_FP64, clat.SCHED_SHARED)
+        for i in inters:
+            eng.initialize(i)
+        eng.set_atoms(w["cells"], 4)
+        for s in w["states"]:
+            eng.add_state(s)
+        return eng
+
+    ref = engine([last])
+    want_q, want_p = ref.compute_integrals(quads), ref.potential_at_atoms(3)
+    on_cosets = "cosets" in ref.kernels()
+    assert on_cosets == (box == (16, 16, 16))
+    ref.close()
+    # two interactions before any run: only the last is built
+    a = engine([first, last])
+    assert np.array_equal(a.compute_integrals(quads), want_q)
+    a.close()
+    # the table built by a potential run first
+    b = engine([last])
+    assert np.array_equal(b.potential_at_atoms(3), want_p)
+    assert np.array_equal(b.compute_integrals(quads), want_q)
+    # a new interaction after runs, then the old one again
+    b.initialize(first)
+    assert not np.allclose(b.compute_integrals(quads), want_q)
+    b.initialize(last)
+    assert np.array_equal(b.compute_integrals(quads), want_q)
+    b.close()
