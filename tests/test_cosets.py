@@ -176,3 +176,39 @@ def test_half_box_matches_oracle(dims):
     Vo = O.prepare(F, Gs)
     v0, _ = half_box(F)
     assert np.max(np.abs(v0 - Vo[0::2, 0::2, 0::2])[m0]) > 1e-6 * scale
+
+
+def test_half_length_lines():
+    """The x passes on half-length lines (linepass.hpp, HL): on the coarse
+    grid's fcc sublattice every site of a line (y, z) has x = 2j + p with
+    p = (y + z) mod 2, so the stored frequencies f < X of the line's 2X-point
+    transform are W_2X^{f p} times the 2(X/2)-point transform of the X/2
+    points j, and the inverse of frequencies f < X at x = 2j + p is the
+    X/2-point line's inverse of the spectrum times conj(W_2X^{f p}).  Checked
+    on random lines of every parity, and the parity rule on zincblende's own
+    coarse positions; the other parity's phase fails."""
+    import zincblende as zb
+    rng = np.random.default_rng(5)
+    for X in (16, 20, 64):
+        H = X // 2
+        W2X = np.exp(-2j * np.pi * np.arange(X) / (2 * X))   # W_2X^f, f < X
+        f = np.arange(X)
+        for p in (0, 1):
+            u = np.zeros(X, complex)
+            u[p::2] = rng.standard_normal(H) + 1j * rng.standard_normal(H)
+            full = np.fft.fft(np.concatenate([u, np.zeros(X)]))[:X]           # 2X-point, f < X
+            half = W2X ** p * np.fft.fft(np.concatenate([u[p::2], np.zeros(H)]))  # 2H = X points
+            assert np.max(np.abs(half - full)) < 1e-12 * np.max(np.abs(full))
+            wrong = W2X ** (1 - p) * np.fft.fft(np.concatenate([u[p::2], np.zeros(H)]))
+            assert np.max(np.abs(wrong - full)) > 1e-3 * np.max(np.abs(full))
+            S = rng.standard_normal(X) + 1j * rng.standard_normal(X)            # frequencies f < X
+            inv_full = np.fft.ifft(np.concatenate([S, np.zeros(X)])) * (2 * X)  # unnormalised, 2X points
+            inv_half = np.fft.ifft(S * np.conj(W2X ** p)) * X                   # X = 2H points, first H
+            assert np.max(np.abs(inv_half[:H] - inv_full[p:X:2])) < 1e-12 * np.max(np.abs(inv_full))
+        assert np.allclose(W2X[f] ** 2, np.exp(-2j * np.pi * f / X))
+    # the parity rule on zincblende's coarse positions: both cosets' sites
+    # (2m and 2m + (1,1,1)) have m_x = m_y + m_z (mod 2)
+    idx = zb.grid_indices(4, 6, 8)
+    assert np.all((idx[:, 0] % 2 == idx[:, 1] % 2) & (idx[:, 1] % 2 == idx[:, 2] % 2))
+    m = idx // 2
+    assert np.all(m[:, 0] % 2 == (m[:, 1] + m[:, 2]) % 2)
