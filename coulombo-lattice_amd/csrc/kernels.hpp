@@ -460,10 +460,10 @@ __global__ void k_coset_gtable(GArgs g, int Xi, int Yi, int Zi, double* __restri
   const int Xc = Xi / 2, Yc = Yi / 2, Zc = Zi / 2;
   const size_t n = (size_t)(Xc + 1) * (Yc + 1) * (Zc + 1);
   for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < n; t += (size_t)gridDim.x * blockDim.x) {
-    const int dx = (int)(t % (Xc + 1));
-    const size_t u = t / (Xc + 1);
-    const int dy = (int)(u % (Yc + 1));
-    const int dz = (int)(u / (Yc + 1));
+    const int dx = (int)((uint32_t)t % (uint32_t)(Xc + 1));  // 32-bit: fewer than 2^32 entries
+    const uint32_t u = (uint32_t)t / (uint32_t)(Xc + 1);
+    const int dy = (int)(u % (uint32_t)(Yc + 1));
+    const int dz = (int)(u / (uint32_t)(Yc + 1));
     G0[t] = gval(g, 2 * dx, 2 * dy, 2 * dz);
     Gt[t] = (dx < Xc && dy < Yc && dz < Zc) ? gval(g, 2 * dx + 1, 2 * dy + 1, 2 * dz + 1) : 0.0;
   }
@@ -475,8 +475,8 @@ __global__ void k_coset_gtable(GArgs g, int Xi, int Yi, int Zi, double* __restri
 // s = (Xc, Yc, Zc), so only the frequencies kx < Xc are carried, against the
 // kernel summed over the shift, K(k) + K(k + s).  Columns kx in [Xc, W) (W
 // rounded up to the strip width) hold zero.
-__device__ __forceinline__ int refl(int k, int n2) {  // k mod n2, reflected into [0, n2/2]
-  k %= n2;
+__device__ __forceinline__ int refl(int k, int n2) {  // k mod n2 for k in [0, 2 n2), reflected into [0, n2/2]
+  if (k >= n2) k -= n2;
   return k <= n2 / 2 ? k : n2 - k;
 }
 
@@ -487,13 +487,14 @@ __global__ void k_gconv_tiled_half(const double* __restrict__ G0, int Xc, int Yc
                                    T* __restrict__ out) {
   const size_t n = (size_t)(Yc + 1) * ncb * (Zc + 1) * C;
   for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < n; t += (size_t)gridDim.x * blockDim.x) {
-    const int c = (int)(t % C);
-    size_t u = t / C;
-    const int rz = (int)(u % (Zc + 1));
-    u /= (Zc + 1);
-    const int blk = (int)(u % ncb);
-    const int ry = (int)(u / ncb);
-    const int kx = blk * C + c;
+    // 32-bit index arithmetic: the table has fewer than 2^32 entries
+    const uint32_t c = (uint32_t)t % (uint32_t)C;
+    uint32_t u = (uint32_t)t / (uint32_t)C;
+    const int rz = (int)(u % (uint32_t)(Zc + 1));
+    u /= (uint32_t)(Zc + 1);
+    const int blk = (int)(u % (uint32_t)ncb);
+    const int ry = (int)(u / (uint32_t)ncb);
+    const int kx = blk * C + (int)c;
     double v = 0.0;
     if (kx < Xc) {
       const size_t X1 = (size_t)Xc + 1, Y1 = (size_t)Yc + 1;
@@ -524,17 +525,18 @@ __global__ void k_coarse_q_half(const double* __restrict__ St, const double2* __
   const int Z2 = 2 * Zc, Y2 = 2 * Yc;
   const size_t n = (size_t)Y2 * ncb * Z2 * C;
   for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < n; t += (size_t)gridDim.x * blockDim.x) {
-    const int c = (int)(t % C);
-    size_t u = t / C;
-    const int fz = (int)(u % Z2);
-    u /= Z2;
-    const int blk = (int)(u % ncb);
-    const int fy = (int)(u / ncb);
-    const int kx = blk * C + c;
+    const uint32_t c = (uint32_t)t % (uint32_t)C;  // 32-bit: fewer than 2^32 entries
+    uint32_t u = (uint32_t)t / (uint32_t)C;
+    const int fz = (int)(u % (uint32_t)Z2);
+    u /= (uint32_t)Z2;
+    const int blk = (int)(u % (uint32_t)ncb);
+    const int fy = (int)(u / (uint32_t)ncb);
+    const int kx = blk * C + (int)c;
     C2<T> v = mk<T>((T)0, (T)0);
     if (kx < Xc) {
       const double2 a = coarse_gt_at(St, ph, Xc, Yc, Zc, kx, fy, fz);
-      const double2 b = coarse_gt_at(St, ph, Xc, Yc, Zc, kx + Xc, (fy + Yc) % Y2, (fz + Zc) % Z2);
+      const double2 b = coarse_gt_at(St, ph, Xc, Yc, Zc, kx + Xc, fy + Yc < Y2 ? fy + Yc : fy - Yc,
+                                     fz + Zc < Z2 ? fz + Zc : fz - Zc);
       v = mk<T>((T)(a.x + b.x), (T)(a.y + b.y));
     }
     Q[t] = v;
