@@ -153,26 +153,32 @@ def test_cosets_batches_bit_identical(monkeypatch):
 
 @pytest.mark.parametrize("prec", [clat.PREC_FP64, clat.PREC_FP32X], ids=["fp64", "fp32x"])
 def test_coset_table_routes_agree(prec, monkeypatch):
-    """The coarse tables built from the transformed fine table (interaction
-    set before the atoms: the eight half-box shifts) and from the real-space
-    samples (interaction set again with the atoms in place: coarse DCT-I and
-    DCT-II) give the same integrals.  The dense protocol (set_input, prepare,
-    calculate) on the same engine afterwards builds the fine table it needs."""
+    """The coarse tables built from the transformed fine table (the dense
+    protocol ran first, so the fine DCT-I exists when the atoms arrive: the
+    eight half-box shifts) and from the real-space samples (an interaction
+    set with the atoms in place, built at the next run: coarse DCT-I and
+    DCT-II) give the same integrals; the dense protocol's value matches the
+    oracle."""
     nh, ne = 2, 1
     w = zb.workload(16, 16, 16, nh, ne, config=6, sigma_frac=0.2, n_orb=4)
     quads = clat.pattern_enumerate("****", nh + ne, nh)
-    eng = _engine(w, w["cells"], prec, 4, w["states"], True, monkeypatch)
-    assert "cosets" in eng.kernels()
-    shifted = eng.compute_integrals(quads)
-    eng.initialize(clat.InteractionSimple(w["step"], 0.0, 10.89))
-    sampled = eng.compute_integrals(quads)
+    monkeypatch.delenv("CLAT_COSETS", raising=False)
     X, Y, Z = w["dims"]
+    eng = clat.Engine((X, Y, Z), prec, clat.SCHED_SHARED)
+    eng.initialize(clat.InteractionSimple(w["step"], 0.0, 10.89))
     F = np.zeros((Z, Y, X), np.complex128)
     F[3, 5, 7], F[40, 22, 9], F[61, 60, 2] = 1.0 + 0.5j, -0.7, 0.25j
     clat._check(clat.lib().clat_set_input(eng.handle, F.view(np.float64).reshape(-1)))
     clat._check(clat.lib().clat_prepare(eng.handle))
     out = np.zeros(2)
     clat._check(clat.lib().clat_calculate(eng.handle, out))
+    eng.set_atoms(w["cells"], 4)
+    for c in w["states"]:
+        eng.add_state(c)
+    assert "cosets" in eng.kernels()
+    shifted = eng.compute_integrals(quads)
+    eng.initialize(clat.InteractionSimple(w["step"], 0.0, 10.89))
+    sampled = eng.compute_integrals(quads)
     eng.close()
     Gh = O.ghat(O.gtable(X, Y, Z, w["step"], 0.0, 10.89))
     dense_want = O.calculate(O.prepare(F, Gh), F)
@@ -181,9 +187,11 @@ def test_coset_table_routes_agree(prec, monkeypatch):
     if prec == clat.PREC_FP64:
         assert np.max(rel_err(sampled, shifted)) < 1e-12
         check(sampled, want, prec, quads)
+        check(shifted, want, prec, quads)
     else:
         assert np.max(rel_err(sampled, shifted)) < 1e-5
         check_fp32x_model(sampled, want, quads, "sampled tables fp32x")
+        check_fp32x_model(shifted, want, quads, "shifted tables fp32x")
 
 
 @pytest.mark.parametrize("prec", [clat.PREC_FP64, clat.PREC_FP32X], ids=["fp64", "fp32x"])
