@@ -694,6 +694,7 @@ void launch_line(clat_engine* e, const void* in, void* out, long long bs, int ki
   // FP32X forward keeps full lines (see xhalf)
   const bool hl = e->xhalf && (MODE == PM_INV || e->rprec != CLAT_PREC_FP32X);
   L.hl = hl;
+  L.ho = !hl && e->fmax == e->Xi;  // half box on full lines: the forward forms f < Xi only
   if (hl) {
     L.T2L = sizeof(TC) == 8 ? e->tw64[0] : e->tw32[0];
     L.PH = sizeof(TC) == 8 ? (const void*)e->dtw[0] : (const void*)e->dtw32[0];

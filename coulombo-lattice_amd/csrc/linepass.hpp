@@ -117,16 +117,21 @@ constexpr size_t line_lds() {
   return (size_t)C * NSUB * R2 * line_r1p<R1>() * sizeof(C2<TC>);
 }
 
-template <typename TI, typename TC, typename TO, int MODE, int R1, int R2, int NSUB, int C, bool HL>
+// LV: 0 plain lines; 1 half-length lines (HL); 2 the forward's stored
+// frequencies f < L only (the half box on full lines, R1 even: outputs
+// k1 >= R1 / 2 of the R1-point DFTs are never formed, the rest bit-identical)
+template <typename TI, typename TC, typename TO, int MODE, int R1, int R2, int NSUB, int C, int LV>
 // the complex64 forward held to six waves per SIMD (80 VGPRs) where that
-// spills nothing (R1 <= 16): the 16 x 8 x forward of the 1M-atom grid from
-// 89 VGPRs, 0.285-0.291 -> 0.271-0.274 ms per step
-__attribute__((amdgpu_waves_per_eu((MODE == PM_FWD && sizeof(TC) == 4 && R1 <= 16) ? 6 : 1)))
+// spills nothing (R1 <= 16, not on half-length lines): the 16 x 8 x forward
+// of the 1M-atom grid from 89 VGPRs, 0.285-0.291 -> 0.271-0.274 ms per step
+__attribute__((amdgpu_waves_per_eu((MODE == PM_FWD && sizeof(TC) == 4 && R1 <= 16 && LV != 1) ? 6 : 1)))
 __global__ __launch_bounds__((line_threads<C, R1, R2, NSUB>())) void k_linepass(const C2<TI>* in, C2<TO>* out,
                                                                                 LineArgs a,
                                                                                 const C2<TC>* __restrict__ T2L,
                                                                                 const C2<TC>* __restrict__ PH) {
   using CT = C2<TC>;
+  constexpr bool HL = LV == 1;
+  constexpr int K1N = (LV == 2 && MODE == PM_FWD && R1 % 2 == 0) ? R1 / 2 : R1;  // forward outputs formed
   constexpr int L = R1 * R2, R1P = line_r1p<R1>();
   constexpr int NTH = line_threads<C, R1, R2, NSUB>();
   constexpr uint32_t SC = sizeof(CT);
@@ -220,7 +225,7 @@ __global__ __launch_bounds__((line_threads<C, R1, R2, NSUB>())) void k_linepass(
         }
       }
 #pragma unroll
-      for (int k1 = 0; k1 < R1; ++k1) {
+      for (int k1 = 0; k1 < K1N; ++k1) {
         const uint32_t o = f0 + k1 * R2 * NSUB < a.fmax ? (uint32_t)(lb + off[k1]) * (uint32_t)sizeof(C2<TO>) : COL_OOB;
         stg_buf<ST_XFWD>(ob, o, cvt<TC, TO>(w[k1]));
       }
@@ -319,6 +324,7 @@ struct LineLaunch {
   const void* T2L;
   const void* PH;  // half-length lines: W_4L^f, f < 2L (else null)
   bool hl;         // half-length lines
+  bool ho;         // forward on full lines storing f < L only (fmax = L)
 };
 
 template <typename TI, typename TC, typename TO, int MODE, int R1, int R2>
@@ -329,11 +335,15 @@ hipError_t line_go(const LineLaunch& L) {
   constexpr size_t lds = line_lds<TC, R1, R2, NSUB, C>();
   static_assert(lds <= 65536, "line pass tile exceeds 64 KiB of LDS");
   if (L.hl)
-    hipLaunchKernelGGL((k_linepass<TI, TC, TO, MODE, R1, R2, NSUB, C, true>), dim3(L.blocks, L.nb), dim3(nth), lds,
+    hipLaunchKernelGGL((k_linepass<TI, TC, TO, MODE, R1, R2, NSUB, C, 1>), dim3(L.blocks, L.nb), dim3(nth), lds,
                        L.stream, (const C2<TI>*)L.in, (C2<TO>*)L.out, L.a, (const C2<TC>*)L.T2L,
                        (const C2<TC>*)L.PH);
+  else if (MODE == PM_FWD && R1 % 2 == 0 && L.ho)
+    hipLaunchKernelGGL((k_linepass<TI, TC, TO, MODE, R1, R2, NSUB, C, MODE == PM_FWD ? 2 : 0>), dim3(L.blocks, L.nb),
+                       dim3(nth), lds, L.stream, (const C2<TI>*)L.in, (C2<TO>*)L.out, L.a, (const C2<TC>*)L.T2L,
+                       (const C2<TC>*)L.PH);
   else
-    hipLaunchKernelGGL((k_linepass<TI, TC, TO, MODE, R1, R2, NSUB, C, false>), dim3(L.blocks, L.nb), dim3(nth), lds,
+    hipLaunchKernelGGL((k_linepass<TI, TC, TO, MODE, R1, R2, NSUB, C, 0>), dim3(L.blocks, L.nb), dim3(nth), lds,
                        L.stream, (const C2<TI>*)L.in, (C2<TO>*)L.out, L.a, (const C2<TC>*)L.T2L,
                        (const C2<TC>*)L.PH);
   return hipGetLastError();
