@@ -274,3 +274,27 @@ def test_half_lines_match_oracle(prec, box, monkeypatch):
         check(ga, want, prec, quads)
         check(gb, want, prec, quads)
 
+
+
+@pytest.mark.parametrize("prec", [clat.PREC_FP64, clat.PREC_FP32X], ids=["fp64", "fp32x"])
+def test_half_box_odd_radix(prec, monkeypatch):
+    """144x64x64 zincblende grid: coarse x lines of 72 points (9 x 8, an odd
+    first radix, so the half-box forward stores its frequencies below 72
+    through out-of-range offsets instead of leaving outputs unformed, in a
+    plane of 80 columns) and half-length lines of 36 points (6 x 6) on the
+    x inverse.  Integrals against the oracle."""
+    nh, ne = 2, 1
+    w = zb.workload(36, 16, 16, nh, ne, config=0, sigma_frac=0.2, n_orb=4)
+    X, Y, Z = w["dims"]
+    assert (X, Y, Z) == (144, 64, 64)
+    quads = clat.pattern_enumerate("****", nh + ne, nh)
+    print("case half box odd radix", prec, w["dims"], flush=True)
+    eng = _engine(w, w["cells"], prec, 4, w["states"], True, monkeypatch)
+    assert {"cosets", "half_box", "half_lines"} <= eng.kernels(), eng.kernels()
+    got = eng.compute_integrals(quads)
+    eng.close()
+    want = _oracle(w, w["cells"], w["states"], quads, nh + ne)
+    if prec == clat.PREC_FP64:
+        check(got, want, prec, quads)
+    else:
+        check_fp32x_model(got, want, quads, "odd radix fp32x")
